@@ -1,0 +1,403 @@
+"""Python binding of libbpx.so, the gfx950 blastp word finder (include/bpx.h).
+
+The host-side mirror of the reference's GPU boundary
+(ncbi_blast_files/gpu_cpu_common.h:40-98): a database volume is loaded once
+into a resident device arena (bpx_db_load), a query's lookup table is built
+and staged once (bpx_query_prepare), and a scan runs the word finder over a
+range of subjects and returns the GPUBlastInitHitList-shaped records plus the
+per-subject d_Hits counters (gpu_blastp_kernel.cu:264-276, 310-312).
+
+Everything here is plumbing over the C ABI; the compute is in the HIP
+kernels.  Importing this module never falls back to a CPU path: if the
+shared library is missing the import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libbpx.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run __graft_entry__.build() "
+        "(make -C blastp-cuda_amd) first")
+_lib = C.CDLL(LIB_PATH)
+
+ALPHABET = 28
+STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
+
+OUT_SUBJECT_STATS = 1
+
+HSP_DTYPE = np.dtype([("hsp_q", "<i4"), ("hsp_s", "<i4"), ("query_offset", "<i4"),
+                      ("subject_offset", "<i4"), ("hsp_len", "<i4"), ("score", "<i4")])
+STATS_DTYPE = np.dtype([("hits", "<i4"), ("successful_extensions", "<i4"),
+                        ("hits_extended", "<i4")])
+
+
+class BpxError(RuntimeError):
+    pass
+
+
+class WordParams(C.Structure):
+    _fields_ = [("word_length", C.c_int32), ("threshold", C.c_int32), ("window", C.c_int32),
+                ("x_dropoff", C.c_int32), ("cutoff_score", C.c_int32), ("diag_mask", C.c_int32)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("waves_per_group", C.c_int32), ("groups_per_cu", C.c_int32),
+                ("subjects_per_grab", C.c_int32), ("queue_capacity", C.c_int32)]
+
+
+class UngappedStats(C.Structure):
+    _fields_ = [("lookup_hits", C.c_int64), ("num_seqs_lookup_hits", C.c_int64),
+                ("init_extends", C.c_int64), ("good_init_extends", C.c_int64),
+                ("num_seqs_passed", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_subjects", C.c_uint32),
+                ("len_median", C.c_double), ("len_sigma", C.c_double),
+                ("len_min", C.c_uint32), ("len_max", C.c_uint32),
+                ("homolog_count", C.c_uint32), ("homolog_fraction", C.c_double),
+                ("identity_min", C.c_double), ("identity_max", C.c_double),
+                ("indel_min", C.c_double), ("indel_max", C.c_double),
+                ("biased_fraction", C.c_double), ("dirichlet_alpha", C.c_double),
+                ("motif_min", C.c_uint32), ("motif_max", C.c_uint32)]
+
+
+_vp = C.c_void_p
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _sig(name, res, *args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("bpx_available", C.c_int)
+_sig("bpx_last_error", C.c_char_p)
+_sig("bpx_context_create", C.c_int, C.c_int, C.POINTER(_vp))
+_sig("bpx_context_destroy", C.c_int, _vp)
+_sig("bpx_context_info", C.c_int, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+_sig("bpx_db_load", C.c_int, _vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(_vp))
+_sig("bpx_db_info", C.c_int, _vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+_sig("bpx_db_destroy", C.c_int, _vp)
+_sig("bpx_query_prepare", C.c_int, _vp, _vp, C.c_int32, _vp, C.POINTER(WordParams), C.POINTER(_vp))
+_sig("bpx_query_info", C.c_int, _vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+     C.POINTER(C.c_int64))
+_sig("bpx_query_destroy", C.c_int, _vp)
+_sig("bpx_check_memory", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch),
+     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
+_sig("bpx_scan", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
+     C.POINTER(_vp))
+_sig("bpx_result_wait", C.c_int, _vp)
+_sig("bpx_result_timing", C.c_int, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float))
+_sig("bpx_result_summary", C.c_int, _vp, C.POINTER(C.c_uint64), C.POINTER(UngappedStats))
+_sig("bpx_result_records", C.c_int, _vp, _vp, _vp, C.c_uint64)
+_sig("bpx_result_subject_stats", C.c_int, _vp, _vp, C.c_uint32)
+_sig("bpx_result_destroy", C.c_int, _vp)
+_sig("bpx_blosum62", C.POINTER(C.c_int8))
+_sig("bpx_table_build", C.c_int, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_uint64,
+     C.POINTER(C.c_uint64))
+_sig("bpx_synth_query", C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, _vp)
+_sig("bpx_synth_query_lengths", C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_double,
+     C.c_double, C.c_double, _vp)
+_sig("bpx_synth_db_generate", C.c_int, C.POINTER(SynthSpec), _vp, _vp, C.c_uint32, C.c_int,
+     C.POINTER(_vp))
+_sig("bpx_synth_db_arrays", C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+     C.POINTER(C.c_uint32), C.POINTER(C.c_uint64))
+_sig("bpx_synth_db_kinds", C.c_int, _vp, C.POINTER(_vp))
+_sig("bpx_synth_db_sha256", C.c_int, _vp, C.c_char_p)
+_sig("bpx_synth_db_free", C.c_int, _vp)
+_sig("bpx_sha256_hex", C.c_int, _vp, C.c_uint64, C.c_char_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise BpxError(f"bpx error {rc}: {_lib.bpx_last_error().decode(errors='replace')}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def available() -> bool:
+    return bool(_lib.bpx_available())
+
+
+def blosum62() -> np.ndarray:
+    p = _lib.bpx_blosum62()
+    return np.ctypeslib.as_array(p, shape=(ALPHABET * ALPHABET,)).copy().reshape(ALPHABET, ALPHABET)
+
+
+def encode(seq: str) -> np.ndarray:
+    """Letters -> ncbistdaa codes."""
+    table = {c: i for i, c in enumerate(STDAA)}
+    return np.array([table[c] for c in seq.upper()], dtype=np.uint8)
+
+
+def default_params(qlen: int | None = None, **kw) -> WordParams:
+    """blastp defaults: word 3, T 11, window 40, X-drop 16, cutoff 41 (raw);
+    diag_mask 0 lets the library derive next_pow2(qlen + window) - 1."""
+    p = WordParams(word_length=3, threshold=11, window=40, x_dropoff=16, cutoff_score=41, diag_mask=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def table_build(query: np.ndarray, threshold=11, word_length=3, matrix=None):
+    """Host-side lookup table as (cell_start, entries) CSR over all words."""
+    q = np.ascontiguousarray(query, dtype=np.uint8)
+    m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.int8)
+    nwords = 1 << (5 * word_length)
+    cs = np.zeros(nwords + 1, dtype=np.uint32)
+    n = C.c_uint64(0)
+    rc = _lib.bpx_table_build(_ptr(q), len(q), _ptr(m), threshold, word_length, _ptr(cs), None, 0,
+                              C.byref(n))
+    ent = np.zeros(max(int(n.value), 1), dtype=np.uint16)
+    _check(_lib.bpx_table_build(_ptr(q), len(q), _ptr(m), threshold, word_length, _ptr(cs), _ptr(ent),
+                                len(ent), C.byref(n)))
+    del rc
+    return cs, ent[: int(n.value)]
+
+
+# ---- synthetic workloads ----------------------------------------------------
+
+def synth_query(seed: int, index: int, length: int) -> np.ndarray:
+    out = np.zeros(max(length, 1), dtype=np.uint8)
+    _check(_lib.bpx_synth_query(seed, index, length, _ptr(out)))
+    return out[:length]
+
+
+def synth_query_lengths(seed: int, n: int, kind: str, lo: float, hi: float, median=0.0, sigma=0.0):
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    _check(_lib.bpx_synth_query_lengths(seed, n, 0 if kind == "loguniform" else 1, lo, hi, median, sigma,
+                                        _ptr(out)))
+    return out[:n]
+
+
+def sha256_hex(data: np.ndarray) -> str:
+    a = np.ascontiguousarray(data)
+    buf = C.create_string_buffer(65)
+    _check(_lib.bpx_sha256_hex(_ptr(a), a.nbytes, buf))
+    return buf.value.decode()
+
+
+@dataclass
+class SynthDB:
+    residues: np.ndarray   # uint8, packed back to back
+    offsets: np.ndarray    # uint64, num_subjects + 1
+    lengths: np.ndarray    # uint32
+    kinds: np.ndarray      # uint8: 0 background, 1 homolog, 2 biased
+    sha256: str
+
+    @property
+    def num_subjects(self) -> int:
+        return int(self.lengths.shape[0])
+
+    @property
+    def num_residues(self) -> int:
+        return int(self.residues.shape[0])
+
+
+def synth_db(spec: dict, queries=(), nthreads: int = 0) -> SynthDB:
+    s = SynthSpec()
+    for k, _ in SynthSpec._fields_:
+        if k in spec:
+            setattr(s, k, spec[k])
+    qs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+    qptrs = (C.c_void_p * max(len(qs), 1))(*[q.ctypes.data for q in qs])
+    qlens = np.array([len(q) for q in qs] or [0], dtype=np.uint32)
+    h = _vp()
+    _check(_lib.bpx_synth_db_generate(C.byref(s), qptrs, _ptr(qlens), len(qs), nthreads, C.byref(h)))
+    try:
+        res, off, ln, kd = _vp(), _vp(), _vp(), _vp()
+        n, nres = C.c_uint32(), C.c_uint64()
+        _check(_lib.bpx_synth_db_arrays(h, C.byref(res), C.byref(off), C.byref(ln), C.byref(n),
+                                        C.byref(nres)))
+        _check(_lib.bpx_synth_db_kinds(h, C.byref(kd)))
+        n, nres = int(n.value), int(nres.value)
+
+        def arr(p, count, dt):
+            if count == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(count,)).copy()
+
+        buf = C.create_string_buffer(65)
+        _check(_lib.bpx_synth_db_sha256(h, buf))
+        return SynthDB(arr(res, nres, np.uint8), arr(off, n + 1, np.uint64), arr(ln, n, np.uint32),
+                       arr(kd, n, np.uint8), buf.value.decode())
+    finally:
+        _lib.bpx_synth_db_free(h)
+
+
+# ---- device objects -----------------------------------------------------------
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._h = _vp()
+        _check(_lib.bpx_context_create(device, C.byref(self._h)))
+
+    def info(self):
+        cus, lds, hbm = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(_lib.bpx_context_info(self._h, C.byref(cus), C.byref(lds), C.byref(hbm)))
+        return {"num_cus": cus.value, "lds_bytes": lds.value, "hbm_bytes": hbm.value}
+
+    def close(self):
+        if self._h:
+            _lib.bpx_context_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_db(self, residues, offsets, lengths) -> "Database":
+        return Database(self, residues, offsets, lengths)
+
+    def prepare_query(self, query, params: WordParams | None = None, matrix=None) -> "Query":
+        return Query(self, query, params, matrix)
+
+
+class Database:
+    def __init__(self, ctx: Context, residues, offsets, lengths):
+        self.ctx = ctx
+        r = np.ascontiguousarray(residues, dtype=np.uint8)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+        self.num_subjects = int(ln.shape[0])
+        self._h = _vp()
+        _check(_lib.bpx_db_load(ctx._h, _ptr(r), _ptr(o), _ptr(ln), self.num_subjects, C.byref(self._h)))
+        n, nres, ab = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(_lib.bpx_db_info(self._h, C.byref(n), C.byref(nres), C.byref(ab)))
+        self.num_residues = int(nres.value)
+        self.arena_bytes = int(ab.value)
+
+    def close(self):
+        if self._h:
+            _lib.bpx_db_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Query:
+    def __init__(self, ctx: Context, query, params: WordParams | None, matrix):
+        self.ctx = ctx
+        self.seq = np.ascontiguousarray(query, dtype=np.uint8)
+        self.params = params if params is not None else default_params()
+        m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.int8)
+        self._h = _vp()
+        _check(_lib.bpx_query_prepare(ctx._h, _ptr(self.seq), len(self.seq), _ptr(m),
+                                      C.byref(self.params), C.byref(self._h)))
+
+    def info(self):
+        tb, lr, npw, ne = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
+        _check(_lib.bpx_query_info(self._h, C.byref(tb), C.byref(lr), C.byref(npw), C.byref(ne)))
+        return {"table_bytes": tb.value, "lds_resident": bool(lr.value), "present_words": npw.value,
+                "entries": ne.value}
+
+    def close(self):
+        if self._h:
+            _lib.bpx_query_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _launch(launch):
+    if launch is None:
+        return None
+    if isinstance(launch, Launch):
+        return launch
+    return Launch(**launch)
+
+
+class Result:
+    """Records (ordered by subject, then subject/query offset), their owner
+    subjects, per-subject counters (if requested), query-level stats."""
+
+    def __init__(self, handle, first, count, flags):
+        self._h = handle
+        self.first, self.count, self.flags = first, count, flags
+
+    def wait(self) -> "Result":
+        _check(_lib.bpx_result_wait(self._h))
+        return self
+
+    def timing(self):
+        k, d = C.c_float(), C.c_float()
+        _check(_lib.bpx_result_timing(self._h, C.byref(k), C.byref(d)))
+        return {"kernel_ms": k.value, "device_ms": d.value}
+
+    def summary(self):
+        n, st = C.c_uint64(), UngappedStats()
+        _check(_lib.bpx_result_summary(self._h, C.byref(n), C.byref(st)))
+        return int(n.value), st.as_dict()
+
+    def records(self):
+        n, _ = self.summary()
+        rec = np.zeros(max(n, 1), dtype=HSP_DTYPE)
+        sub = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(_lib.bpx_result_records(self._h, _ptr(rec), _ptr(sub), len(rec)))
+        return rec[:n], sub[:n]
+
+    def subject_stats(self) -> np.ndarray:
+        st = np.zeros(max(self.count, 1), dtype=STATS_DTYPE)
+        _check(_lib.bpx_result_subject_stats(self._h, _ptr(st), len(st)))
+        return st[: self.count]
+
+    def close(self):
+        if self._h:
+            _lib.bpx_result_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | None = None,
+         launch=None, subject_stats: bool = True) -> Result:
+    """Asynchronous scan of subjects [first, first+count); call .wait()."""
+    if count is None:
+        count = db.num_subjects - first
+    flags = OUT_SUBJECT_STATS if subject_stats else 0
+    h = _vp()
+    lp = _launch(launch)
+    _check(_lib.bpx_scan(ctx._h, query._h, db._h, first, count, C.byref(lp) if lp is not None else None,
+                         flags, C.byref(h)))
+    return Result(h, first, count, flags)
+
+
+def check_memory(ctx: Context, query: Query, db: Database, first=0, count=None, launch=None):
+    if count is None:
+        count = db.num_subjects - first
+    dev, lds = C.c_size_t(), C.c_size_t()
+    lp = _launch(launch)
+    rc = _lib.bpx_check_memory(ctx._h, db._h, query._h, first, count,
+                               C.byref(lp) if lp is not None else None, C.byref(dev), C.byref(lds))
+    return rc, int(dev.value), int(lds.value)

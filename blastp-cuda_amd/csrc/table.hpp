@@ -1,0 +1,52 @@
+// table.hpp -- host-side query lookup table for the gfx950 word finder.
+//
+// The reference consumes NCBI's BlastAaLookupTable in its "smallbone" form:
+// a presence vector plus 12-byte cells of up to 3 inline u16 query offsets
+// and an overflow array (gpu_blastp.cu:45-61, read at
+// gpu_blastp_kernel.cu:182-192).  That backbone is 28,540 cells x 12 B and
+// does not fit in LDS, so this table keeps the same information in a
+// rank-indexed CSR that does:
+//
+//   groups[g]   = {pv bits of words 32g..32g+31, number of present words
+//                  in groups < g}                       (8 B per 32 words)
+//   starts[r]   = first entry of the r-th present word  (u32, npresent+1)
+//   entries[e]  = query offset                          (u16)
+//
+// A word w is present iff bit (w & 31) of groups[w>>5].pv is set; its rank
+// is groups[w>>5].base + popcount(pv & ((1<<(w&31))-1)); its query offsets
+// are entries[starts[rank] .. starts[rank+1]).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace bpx {
+
+constexpr int kAlphabet = 28;
+constexpr int kCharsize = 5;
+
+struct HostTable {
+    int32_t word_length = 0;
+    uint32_t nwords = 0;                // 1 << (5 * word_length)
+    uint32_t ngroups = 0;               // nwords / 32
+    std::vector<uint32_t> groups;       // 2 * ngroups: {pv, base} pairs
+    std::vector<uint32_t> starts;       // npresent + 1
+    std::vector<uint16_t> entries;      // query offsets, ascending per word
+    uint32_t npresent = 0;
+    uint32_t max_cell = 0;
+
+    // Serialises the table into one word-aligned blob:
+    // [groups (8*ngroups B)][starts (4*(npresent+1) B)][entries (2*n B, padded to 4)]
+    std::vector<uint32_t> pack() const;
+    uint64_t packed_bytes() const;
+};
+
+// Builds the neighbourhood table of `query` (ncbistdaa, qlen >= 0): word w is
+// listed under query position p iff sum_i M[q[p+i]][w_i] >= threshold, or w
+// is the query word itself.  Neighbours are enumerated with row-maximum
+// pruning over the 28-letter alphabet, once per distinct query word.
+// Returns false on bad arguments.
+bool build_table(const uint8_t* query, int32_t qlen, const int8_t* matrix28,
+                 int32_t threshold, int32_t word_length, HostTable* out);
+
+}  // namespace bpx

@@ -1,0 +1,188 @@
+/*
+ * bpx.h -- C ABI of the MI355X (gfx950) blastp word finder: two-hit word
+ * finding plus ungapped X-drop extension, the work NCBI's BlastAaWordFinder
+ * does for the preliminary search, run for a whole database volume per call.
+ *
+ * Conventions: every call returns 0 (BPX_OK) or a negative bpx_status;
+ * bpx_last_error() gives the message of the last failure on this thread.
+ * Handles own all device memory.  A handle is used by one host thread at a
+ * time.  No entry point takes or returns framework (torch) types.
+ *
+ * Reference interfaces replaced (avtomaton/blastp-cuda):
+ *   bpx_available        GPU_BLAST_check_availability   gpu_cpu_common.h:91, gpu_blastp.cu:336-344
+ *   bpx_context_*        device selection / properties  gpu_blastp.cu:185-191, 327-329
+ *   bpx_db_load          .gpu volume read + H2D          functions.cpp:620-638, gpu_blastp.cu:139,158
+ *   bpx_query_prepare    lookup/PV/matrix/query staging gpu_blastp.cu:45-88, 140-145, 159-163, 177-183
+ *   bpx_check_memory     GPU_BLASTP_check_memory        gpu_cpu_common.h:81-89, gpu_blastp.cu:282-334
+ *   bpx_scan             GPU_BLASTP (launch)            gpu_cpu_common.h:44-63, gpu_blastp.cu:16-222
+ *   bpx_result_wait      GPU_BLASTP_get_data (sync)     gpu_cpu_common.h:65-69, gpu_blastp.cu:224-259
+ *   bpx_result_*         h_Hits / h_GPUBlastInitHitList consumed at blast_engine.c:520-559
+ *   bpx_*_destroy        GPU_BLASTP_free_memory         gpu_cpu_common.h:71-79, gpu_blastp.cu:262-280
+ */
+#ifndef BPX_H
+#define BPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPX_ALPHABET 28 /* ncbistdaa codes 0..27 */
+
+typedef enum bpx_status {
+    BPX_OK = 0,
+    BPX_E_ARG = -1,       /* bad argument */
+    BPX_E_NODEV = -2,     /* no usable GPU */
+    BPX_E_HIP = -3,       /* HIP runtime error */
+    BPX_E_NOMEM = -4,     /* device or host allocation failed */
+    BPX_E_BUDGET = -5,    /* exceeds the device memory / LDS budget */
+    BPX_E_STATE = -6      /* call out of order (e.g. result read before wait) */
+} bpx_status;
+
+typedef struct bpx_ctx bpx_ctx;
+typedef struct bpx_db bpx_db;
+typedef struct bpx_query bpx_query;
+typedef struct bpx_batch bpx_batch;
+typedef struct bpx_result bpx_result;
+
+/* Word-finding parameters of one query (one NCBI "context").  Mirrors what
+ * the reference reads from the lookup table, diagonal table and
+ * BlastInitialWordParameters (gpu_blastp.cu:63-82):
+ *   word_length  lookup->word_length (2 or 3)
+ *   threshold    neighbourhood threshold T of the lookup table
+ *   window       ewp->diag_table->window (two-hit window A)
+ *   x_dropoff    word_params->cutoffs[ctx].x_dropoff (raw score)
+ *   cutoff_score word_params->cutoffs[ctx].cutoff_score (raw score)
+ *   diag_mask    ewp->diag_table->diag_mask; 0 = derive it as
+ *                (next power of two >= qlen + window) - 1 */
+typedef struct bpx_word_params {
+    int32_t word_length;
+    int32_t threshold;
+    int32_t window;
+    int32_t x_dropoff;
+    int32_t cutoff_score;
+    int32_t diag_mask;
+} bpx_word_params;
+
+/* One ungapped HSP; field-for-field GPUBlastInitHitList (gpu_cpu_common.h:29-38),
+ * i.e. the arguments of BlastSaveInitHsp (blast_engine.c:545-551). */
+typedef struct bpx_init_hsp {
+    int32_t hsp_q;
+    int32_t hsp_s;
+    int32_t query_offset;
+    int32_t subject_offset;
+    int32_t hsp_len;
+    int32_t score;
+} bpx_init_hsp;
+
+/* Per (query, subject) counters: the d_Hits triple of the reference
+ * (gpu_blastp_kernel.cu:310-312), read at blast_engine.c:520-524. */
+typedef struct bpx_subject_stats {
+    int32_t hits;                  /* lookup hits (word matches) */
+    int32_t successful_extensions; /* records saved (score >= cutoff) */
+    int32_t hits_extended;         /* two-hit extensions attempted */
+} bpx_subject_stats;
+
+/* Sums Blast_UngappedStatsUpdate (blast_engine.c:553-556) would accumulate
+ * over every subject of one query. */
+typedef struct bpx_ungapped_stats {
+    int64_t lookup_hits;
+    int64_t num_seqs_lookup_hits;
+    int64_t init_extends;
+    int64_t good_init_extends;
+    int64_t num_seqs_passed;
+} bpx_ungapped_stats;
+
+/* Launch configuration of the scan kernels (the reference's -gpu_blocks /
+ * -gpu_threads, cmdline_flags.cpp:218-222, re-expressed for gfx950).  Any
+ * field left 0 takes the library default or the autotuned value. */
+typedef struct bpx_launch {
+    int32_t waves_per_group;    /* wavefronts per workgroup (1..16) */
+    int32_t groups_per_cu;      /* resident workgroups per CU (grid = CUs x this) */
+    int32_t subjects_per_grab;  /* work items claimed per queue pop */
+    int32_t queue_capacity;     /* per-wave LDS hit queue entries (power of two) */
+} bpx_launch;
+
+/* Output selection for a scan. */
+#define BPX_OUT_SUBJECT_STATS 1u /* copy per-subject counters to the host */
+
+/* ---- device / context ------------------------------------------------- */
+int bpx_available(void);
+int bpx_context_create(int device, bpx_ctx **out);
+int bpx_context_destroy(bpx_ctx *ctx);
+const char *bpx_last_error(void);
+/* Number of compute units and usable LDS bytes per workgroup of the device. */
+int bpx_context_info(const bpx_ctx *ctx, int32_t *num_cus, int64_t *lds_bytes,
+                     int64_t *hbm_bytes);
+
+/* ---- database: a resident int8 residue arena ---------------------------
+ * residues: ncbistdaa codes; subject i is residues[offsets[i] ..
+ * offsets[i]+lengths[i]).  The library repacks it into its own aligned
+ * arena on the device; the host arrays may be freed on return. */
+int bpx_db_load(bpx_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                const uint32_t *lengths, uint32_t num_subjects, bpx_db **out);
+int bpx_db_info(const bpx_db *db, uint32_t *num_subjects, uint64_t *num_residues,
+                uint64_t *arena_bytes);
+int bpx_db_destroy(bpx_db *db);
+
+/* ---- query: lookup table built on the host, uploaded once ---------------
+ * query: ncbistdaa codes, qlen <= 65535.  matrix: 28x28 int8 row-major
+ * (row = query residue) or NULL for the built-in BLOSUM62. */
+int bpx_query_prepare(bpx_ctx *ctx, const uint8_t *query, int32_t qlen,
+                      const int8_t *matrix, const bpx_word_params *params,
+                      bpx_query **out);
+int bpx_query_info(const bpx_query *q, int64_t *table_bytes, int32_t *lds_resident,
+                   int32_t *num_present_words, int64_t *num_entries);
+int bpx_query_destroy(bpx_query *q);
+
+/* Device bytes a scan of subjects [first, first+count) of db would need and
+ * the LDS bytes per workgroup; returns BPX_E_BUDGET if it does not fit. */
+int bpx_check_memory(bpx_ctx *ctx, const bpx_db *db, const bpx_query *q,
+                     uint32_t first, uint32_t count, const bpx_launch *launch,
+                     size_t *device_bytes, size_t *lds_bytes);
+
+/* ---- scan ---------------------------------------------------------------
+ * Asynchronous: enqueues the scan of subjects [first, first+count) on the
+ * context's stream and returns.  launch may be NULL (defaults). */
+int bpx_scan(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, uint32_t first,
+             uint32_t count, const bpx_launch *launch, uint32_t out_flags,
+             bpx_result **out);
+/* Blocks until the scan and its result copies are complete. */
+int bpx_result_wait(bpx_result *r);
+/* Kernel time of the last scan, measured with HIP events on the stream the
+ * kernels run on: the word-finder kernel alone, and the whole device phase
+ * (word finder + compaction + copies to the host). */
+int bpx_result_timing(const bpx_result *r, float *kernel_ms, float *device_ms);
+/* Number of records (all subjects) and the query-level ungapped stats. */
+int bpx_result_summary(const bpx_result *r, uint64_t *num_records,
+                       bpx_ungapped_stats *stats);
+/* Records, ordered by (subject, subject_offset, query_offset); subjects[i]
+ * is the absolute subject index of records[i].  Either pointer may be NULL. */
+int bpx_result_records(const bpx_result *r, bpx_init_hsp *records,
+                       uint32_t *subjects, uint64_t capacity);
+/* Per-subject counters for the scanned range (count entries); available
+ * when the scan was made with BPX_OUT_SUBJECT_STATS. */
+int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
+                             uint32_t capacity);
+int bpx_result_destroy(bpx_result *r);
+
+/* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
+const int8_t *bpx_blosum62(void);
+
+/* Host-only (no device needed): the lookup table bpx_query_prepare builds,
+ * unpacked to CSR over all 2^(5*word_length) word indices -- the query
+ * offsets listed under word w are entries[cell_start[w] .. cell_start[w+1]),
+ * ascending.  Replaces NCBI BlastAaLookupTableNew + BlastAaLookupIndexQuery
+ * (not vendored), whose result the reference reads at gpu_blastp.cu:45-61.
+ * cell_start needs 2^(5*word_length)+1 entries; *num_entries receives the
+ * total even when capacity is too small (then BPX_E_ARG). */
+int bpx_table_build(const uint8_t *query, int32_t qlen, const int8_t *matrix,
+                    int32_t threshold, int32_t word_length, uint32_t *cell_start,
+                    uint16_t *entries, uint64_t capacity, uint64_t *num_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BPX_H */

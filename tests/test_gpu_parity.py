@@ -1,0 +1,269 @@
+"""GPU parity: the HIP word finder (through the C ABI) against the CPU oracle.
+
+Bit-exact on every integer: per-subject counters {hits,
+successful_extensions, hits_extended} and every record field, in the same
+order (subject, then subject offset, then query offset -- the sequential
+emission order).  Workloads: the hand-derived known answers, naive
+straight-line cases, diagonal aliasing, edge lengths, both lookup-table
+placements (LDS and HBM), word length 2, forced record-segment overflow, and
+the full 10,000-subject and 2,000,000-subject single-query databases.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import bpx
+import bpx_letters
+import pyoracle as O
+import ref_straight as R
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STD20 = np.array([1, 16, 13, 4, 3, 15, 5, 7, 8, 9, 11, 10, 12, 6, 14, 17, 18, 20, 22, 19], dtype=np.uint8)
+THREADS = 16
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert bpx.available(), "no HIP device visible"
+    c = bpx.Context(0)
+    say(f"device: {c.info()}")
+    yield c
+    c.close()
+
+
+def pack(subjects):
+    lens = np.array([len(s) for s in subjects], dtype=np.uint32)
+    offs = np.concatenate([[0], np.cumsum(lens.astype(np.uint64))]).astype(np.uint64)
+    res = np.concatenate(subjects).astype(np.uint8) if len(subjects) and lens.sum() else np.zeros(0, np.uint8)
+    return res, offs, lens
+
+
+def gpu_run(ctx, query, res, offs, lens, params=None, matrix=None, first=0, count=None, launch=None):
+    db = ctx.load_db(res, offs[:-1] if len(offs) == len(lens) + 1 else offs, lens)
+    q = ctx.prepare_query(query, params, matrix)
+    r = bpx.scan(ctx, q, db, first, count, launch=launch).wait()
+    recs, subs = r.records()
+    st = r.subject_stats()
+    n, ustats = r.summary()
+    info = q.info()
+    r.close()
+    q.close()
+    db.close()
+    return st, recs, subs, ustats, info
+
+
+def oracle_run(query, res, offs, lens, params=None, matrix=None):
+    prm = None
+    if params is not None:
+        prm = O.params(len(query), params.word_length, params.threshold, params.window, params.x_dropoff,
+                       params.cutoff_score, params.diag_mask)
+    return O.run_db(query, res, offs, lens, prm, THREADS, matrix)
+
+
+def assert_parity(gpu, orc, first=0):
+    st, recs, subs, ustats, _ = gpu
+    counters, recoff, orecs = orc
+    n = len(st)
+    g = np.stack([st["hits"], st["successful_extensions"], st["hits_extended"]], axis=1)
+    np.testing.assert_array_equal(g, counters)
+    assert len(recs) == len(orecs)
+    np.testing.assert_array_equal(recs.view(np.int32).reshape(-1, 6), orecs.view(np.int32).reshape(-1, 6))
+    want_sub = np.repeat(np.arange(first, first + n, dtype=np.uint32),
+                         np.diff(recoff.astype(np.int64)).astype(np.int64))
+    np.testing.assert_array_equal(subs, want_sub)
+    # query-level ungapped stats (Blast_UngappedStatsUpdate sums)
+    has = counters[:, 0] > 0
+    assert ustats["lookup_hits"] == int(counters[has, 0].sum())
+    assert ustats["num_seqs_lookup_hits"] == int(has.sum())
+    assert ustats["init_extends"] == int(counters[has, 2].sum())
+    assert ustats["good_init_extends"] == int(counters[has, 1].sum())
+    assert ustats["num_seqs_passed"] == int((counters[has, 1] > 0).sum())
+
+
+def identity_matrix():
+    M = np.full((28, 28), -4, dtype=np.int8)
+    np.fill_diagonal(M, 5)
+    M[0, :] = -128
+    M[:, 0] = -128
+    return M
+
+
+def test_known_answers(ctx):
+    with open(os.path.join(ROOT, "tests", "golden", "known_answers.json")) as f:
+        cases = json.load(f)["cases"]
+    M = identity_matrix()
+    for case in cases:
+        say(f"known answer: {case['name']}")
+        q = bpx_letters.encode(case["query"])
+        s = bpx_letters.encode(case["subject"])
+        p = bpx.default_params(**{("cutoff_score" if k == "cutoff" else k): v
+                                  for k, v in case.get("params", {}).items()})
+        res, offs, lens = pack([s])
+        st, recs, subs, _, _ = gpu_run(ctx, q, res, offs, lens, p, M)
+        assert [int(st[0]["hits"]), int(st[0]["successful_extensions"]), int(st[0]["hits_extended"])] == \
+            case["counters"], case["name"]
+        assert recs.view(np.int32).reshape(-1, 6).tolist() == case["records"], case["name"]
+
+
+def test_straightline_small(ctx):
+    rng = np.random.default_rng(77)
+    M = O.blosum62()
+    q = rng.choice(STD20, 45)
+    subjects = []
+    for i in range(60):
+        s = rng.choice(STD20, int(rng.integers(0, 150)))
+        if i % 2:
+            seg = q[int(rng.integers(0, 30)):].copy()
+            mut = rng.random(len(seg)) < 0.3
+            seg[mut] = rng.choice(STD20, int(mut.sum()))
+            s = np.concatenate([s, seg])
+        subjects.append(s)
+    res, offs, lens = pack(subjects)
+    p = bpx.default_params(cutoff_score=20)
+    st, recs, subs, _, _ = gpu_run(ctx, q, res, offs, lens, p)
+    k = 0
+    for i, s in enumerate(subjects):
+        h, sv, ex, rr = R.scan(q, s, M, cutoff=20)
+        assert [int(st[i]["hits"]), int(st[i]["successful_extensions"]), int(st[i]["hits_extended"])] == \
+            [h, sv, ex], i
+        got = [tuple(int(x) for x in r) for r in recs[k:k + sv].view(np.int32).reshape(-1, 6)]
+        assert sorted(got) == sorted(rr), i
+        k += sv
+
+
+@pytest.mark.parametrize("diag_len,window", [(4, 5), (8, 40), (16, 12), (64, 40)])
+def test_diagonal_aliasing(ctx, diag_len, window):
+    say(f"aliasing diag_len={diag_len} window={window}")
+    rng = np.random.default_rng(diag_len * 100 + window)
+    q = rng.choice(STD20[:5], 60)
+    subjects = [rng.choice(STD20[:5], int(rng.integers(1, 400))) for _ in range(300)]
+    res, offs, lens = pack(subjects)
+    p = bpx.default_params(window=window, diag_mask=diag_len - 1, cutoff_score=25)
+    assert_parity(gpu_run(ctx, q, res, offs, lens, p), oracle_run(q, res, offs, lens, p))
+
+
+def test_edge_lengths_and_subrange(ctx):
+    rng = np.random.default_rng(5)
+    q = rng.choice(STD20, 120)
+    subjects = [np.zeros(0, np.uint8), rng.choice(STD20, 1), rng.choice(STD20, 2), rng.choice(STD20, 3)]
+    subjects += [np.concatenate([rng.choice(STD20, int(rng.integers(0, 300))), q]) for _ in range(500)]
+    res, offs, lens = pack(subjects)
+    assert_parity(gpu_run(ctx, q, res, offs, lens), oracle_run(q, res, offs, lens))
+    # a sub-range of the volume, as the reference hands the GPU the first
+    # num_sequences_to subjects (functions.cpp:125-149)
+    first, count = 100, 250
+    orc = oracle_run(q, res[offs[first]:offs[first + count]], offs[first:first + count + 1] - offs[first],
+                     lens[first:first + count])
+    assert_parity(gpu_run(ctx, q, res, offs, lens, first=first, count=count), orc, first)
+    # nothing to scan
+    st, recs, subs, ustats, _ = gpu_run(ctx, q, res, offs, lens, first=10, count=0)
+    assert len(st) == 0 and len(recs) == 0 and ustats["lookup_hits"] == 0
+
+
+def test_long_subjects(ctx):
+    """Subjects longer than 65,536 residues exercise the 16-bit ring rebase."""
+    rng = np.random.default_rng(6)
+    q = rng.choice(STD20, 400)
+    subjects = []
+    for n in (35000, 70000, 140001):
+        s = rng.choice(STD20, n)
+        for at in range(1000, n - 500, 9000):
+            s[at:at + 400] = q
+        subjects.append(s)
+    res, offs, lens = pack(subjects)
+    assert_parity(gpu_run(ctx, q, res, offs, lens), oracle_run(q, res, offs, lens))
+
+
+@pytest.mark.parametrize("qlen", [3, 50, 500, 2000, 4000])
+def test_query_lengths_and_table_placement(ctx, qlen):
+    rng = np.random.default_rng(qlen)
+    q = bpx.synth_query(11, qlen, qlen)
+    subjects = [rng.choice(STD20, int(rng.integers(20, 900))) for _ in range(1500)]
+    for i in range(0, 1500, 50):
+        subjects[i] = np.concatenate([subjects[i], q[: min(qlen, 300)]])
+    res, offs, lens = pack(subjects)
+    gpu = gpu_run(ctx, q, res, offs, lens)
+    say(f"qlen {qlen}: table {gpu[4]}")
+    assert_parity(gpu, oracle_run(q, res, offs, lens))
+
+
+def test_word_length_2_and_custom_matrix(ctx):
+    rng = np.random.default_rng(8)
+    M = rng.integers(-4, 7, (28, 28)).astype(np.int8)
+    M[0, :] = -128
+    M[:, 0] = -128
+    q = rng.choice(STD20, 90)
+    subjects = [np.concatenate([rng.choice(STD20, int(rng.integers(0, 200))), q[::-1], q]) for _ in range(400)]
+    res, offs, lens = pack(subjects)
+    for word, T in ((2, 8), (3, 10)):
+        p = bpx.default_params(word_length=word, threshold=T, cutoff_score=30)
+        assert_parity(gpu_run(ctx, q, res, offs, lens, p, M), oracle_run(q, res, offs, lens, p, M))
+
+
+def test_segment_overflow_rescan(ctx):
+    """Every subject carries many high-scoring copies of the query: the
+    per-wave record segments overflow and the library must rescan with
+    larger ones, returning exactly the oracle's records."""
+    rng = np.random.default_rng(9)
+    q = rng.choice(STD20, 60)
+    unit = np.concatenate([q, rng.choice(STD20, 45)])
+    subjects = [np.tile(unit, int(rng.integers(20, 60))) for _ in range(6000)]
+    res, offs, lens = pack(subjects)
+    gpu = gpu_run(ctx, q, res, offs, lens)
+    say(f"overflow case: {len(gpu[1])} records")
+    assert len(gpu[1]) > 6000 * 64 // 10
+    assert_parity(gpu, oracle_run(q, res, offs, lens))
+
+
+@pytest.mark.parametrize("launch", [dict(waves_per_group=1), dict(waves_per_group=4, groups_per_cu=2),
+                                    dict(waves_per_group=16, subjects_per_grab=8),
+                                    dict(queue_capacity=4096)])
+def test_launch_shapes_do_not_change_results(ctx, launch):
+    rng = np.random.default_rng(10)
+    q = bpx.synth_query(3, 0, 350)
+    subjects = [rng.choice(STD20, int(rng.integers(30, 1200))) for _ in range(3000)]
+    for i in range(0, 3000, 30):
+        subjects[i][: min(len(subjects[i]), 200)] = q[: min(len(subjects[i]), 200)]
+    res, offs, lens = pack(subjects)
+    assert_parity(gpu_run(ctx, q, res, offs, lens, launch=launch), oracle_run(q, res, offs, lens))
+
+
+def config_db(which):
+    if which == 1:
+        q = bpx.synth_query(0, 0, 300)
+        spec = dict(seed=0, num_subjects=10_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
+                    homolog_count=50, identity_min=0.5, identity_max=0.5, indel_min=0.02, indel_max=0.02)
+    else:
+        q = bpx.synth_query(0, 0, 500)
+        spec = dict(seed=2, num_subjects=2_000_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
+                    homolog_fraction=0.005, identity_min=0.3, identity_max=0.7, indel_min=0.0, indel_max=0.03)
+    return q, bpx.synth_db(spec, [q], nthreads=THREADS)
+
+
+def test_config1_full(ctx):
+    q, db = config_db(1)
+    say(f"config 1: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
+    gpu = gpu_run(ctx, q, db.residues, db.offsets, db.lengths)
+    orc = oracle_run(q, db.residues, db.offsets, db.lengths)
+    assert_parity(gpu, orc)
+    assert len(orc[2]) > 0
+
+
+def test_config2_full(ctx):
+    q, db = config_db(2)
+    say(f"config 2: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
+    gpu = gpu_run(ctx, q, db.residues, db.offsets, db.lengths)
+    say(f"config 2: GPU done, {len(gpu[1])} records")
+    orc = oracle_run(q, db.residues, db.offsets, db.lengths)
+    say(f"config 2: oracle done, {len(orc[2])} records")
+    assert_parity(gpu, orc)
