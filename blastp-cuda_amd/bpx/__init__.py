@@ -50,7 +50,12 @@ class WordParams(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("waves_per_group", C.c_int32), ("groups_per_cu", C.c_int32),
-                ("subjects_per_grab", C.c_int32), ("queue_capacity", C.c_int32)]
+                ("subjects_per_item", C.c_int32), ("queue_capacity", C.c_int32),
+                ("subject_buffer", C.c_int32), ("reserved", C.c_int32)]
+
+
+class BatchPolicy(C.Structure):
+    _fields_ = [("group_residues", C.c_int32), ("group_queries", C.c_int32)]
 
 
 class UngappedStats(C.Structure):
@@ -103,8 +108,15 @@ _sig("bpx_scan", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launc
 _sig("bpx_result_wait", C.c_int, _vp)
 _sig("bpx_result_timing", C.c_int, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _sig("bpx_result_summary", C.c_int, _vp, C.POINTER(C.c_uint64), C.POINTER(UngappedStats))
-_sig("bpx_result_records", C.c_int, _vp, _vp, _vp, C.c_uint64)
-_sig("bpx_result_subject_stats", C.c_int, _vp, _vp, C.c_uint32)
+_sig("bpx_result_records", C.c_int, _vp, _vp, _vp, _vp, C.c_uint64)
+_sig("bpx_result_subject_stats", C.c_int, _vp, _vp, C.c_uint64)
+_sig("bpx_result_query_stats", C.c_int, _vp, C.POINTER(UngappedStats), C.c_uint32)
+_sig("bpx_batch_prepare", C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(WordParams), _vp, _vp,
+     C.POINTER(BatchPolicy), C.POINTER(_vp))
+_sig("bpx_batch_info", C.c_int, _vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int64))
+_sig("bpx_batch_destroy", C.c_int, _vp)
+_sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
+     C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
 _sig("bpx_blosum62", C.POINTER(C.c_int8))
 _sig("bpx_table_build", C.c_int, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_uint64,
@@ -271,6 +283,10 @@ class Context:
     def prepare_query(self, query, params: WordParams | None = None, matrix=None) -> "Query":
         return Query(self, query, params, matrix)
 
+    def prepare_batch(self, queries, params: WordParams | None = None, matrix=None, x_dropoff=None,
+                      cutoff=None, group_residues=0, group_queries=0) -> "Batch":
+        return Batch(self, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries)
+
 
 class Database:
     def __init__(self, ctx: Context, residues, offsets, lengths):
@@ -326,6 +342,41 @@ class Query:
             pass
 
 
+class Batch:
+    """Many queries staged together (packed into scan groups)."""
+
+    def __init__(self, ctx: Context, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries):
+        self.ctx = ctx
+        self.seqs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+        self.num_queries = len(self.seqs)
+        self.params = params if params is not None else default_params()
+        ptrs = (C.c_void_p * max(self.num_queries, 1))(*[q.ctypes.data for q in self.seqs])
+        lens = np.array([len(q) for q in self.seqs] or [0], dtype=np.int32)
+        m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.int8)
+        xd = None if x_dropoff is None else np.ascontiguousarray(x_dropoff, dtype=np.int32)
+        co = None if cutoff is None else np.ascontiguousarray(cutoff, dtype=np.int32)
+        pol = BatchPolicy(group_residues, group_queries)
+        self._h = _vp()
+        _check(_lib.bpx_batch_prepare(ctx._h, self.num_queries, ptrs, _ptr(lens), _ptr(m), C.byref(self.params),
+                                      _ptr(xd), _ptr(co), C.byref(pol), C.byref(self._h)))
+
+    def info(self):
+        nq, ng, tb = C.c_uint32(), C.c_uint32(), C.c_int64()
+        _check(_lib.bpx_batch_info(self._h, C.byref(nq), C.byref(ng), C.byref(tb)))
+        return {"queries": nq.value, "groups": ng.value, "table_bytes": tb.value}
+
+    def close(self):
+        if self._h:
+            _lib.bpx_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _launch(launch):
     if launch is None:
         return None
@@ -335,12 +386,14 @@ def _launch(launch):
 
 
 class Result:
-    """Records (ordered by subject, then subject/query offset), their owner
-    subjects, per-subject counters (if requested), query-level stats."""
+    """Records (ordered by query, subject, subject offset, query offset),
+    their owner subjects and queries, per-(query, subject) counters (if
+    requested) and per-query ungapped stats."""
 
-    def __init__(self, handle, first, count, flags):
+    def __init__(self, handle, first, count, flags, num_queries=1):
         self._h = handle
         self.first, self.count, self.flags = first, count, flags
+        self.num_queries = num_queries
 
     def wait(self) -> "Result":
         _check(_lib.bpx_result_wait(self._h))
@@ -356,17 +409,27 @@ class Result:
         _check(_lib.bpx_result_summary(self._h, C.byref(n), C.byref(st)))
         return int(n.value), st.as_dict()
 
-    def records(self):
+    def query_stats(self):
+        arr = (UngappedStats * max(self.num_queries, 1))()
+        _check(_lib.bpx_result_query_stats(self._h, arr, len(arr)))
+        return [arr[i].as_dict() for i in range(self.num_queries)]
+
+    def records(self, with_queries=False):
         n, _ = self.summary()
         rec = np.zeros(max(n, 1), dtype=HSP_DTYPE)
         sub = np.zeros(max(n, 1), dtype=np.uint32)
-        _check(_lib.bpx_result_records(self._h, _ptr(rec), _ptr(sub), len(rec)))
+        qry = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(_lib.bpx_result_records(self._h, _ptr(rec), _ptr(sub), _ptr(qry), len(rec)))
+        if with_queries:
+            return rec[:n], sub[:n], qry[:n]
         return rec[:n], sub[:n]
 
     def subject_stats(self) -> np.ndarray:
-        st = np.zeros(max(self.count, 1), dtype=STATS_DTYPE)
+        """Counters per (query, subject), shape (num_queries * count,) query-major."""
+        n = self.num_queries * self.count
+        st = np.zeros(max(n, 1), dtype=STATS_DTYPE)
         _check(_lib.bpx_result_subject_stats(self._h, _ptr(st), len(st)))
-        return st[: self.count]
+        return st[:n]
 
     def close(self):
         if self._h:
@@ -391,6 +454,19 @@ def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | 
     _check(_lib.bpx_scan(ctx._h, query._h, db._h, first, count, C.byref(lp) if lp is not None else None,
                          flags, C.byref(h)))
     return Result(h, first, count, flags)
+
+
+def scan_batch(ctx: Context, batch: Batch, db: Database, first: int = 0, count: int | None = None,
+               launch=None, subject_stats: bool = False) -> Result:
+    """Asynchronous scan of every query of the batch in one persistent launch."""
+    if count is None:
+        count = db.num_subjects - first
+    flags = OUT_SUBJECT_STATS if subject_stats else 0
+    h = _vp()
+    lp = _launch(launch)
+    _check(_lib.bpx_scan_batch(ctx._h, batch._h, db._h, first, count, C.byref(lp) if lp is not None else None,
+                               flags, C.byref(h)))
+    return Result(h, first, count, flags, batch.num_queries)
 
 
 def check_memory(ctx: Context, query: Query, db: Database, first=0, count=None, launch=None):

@@ -1,27 +1,36 @@
 // kernels.hip -- gfx950 kernels of the blastp word finder.
 //
-// k_scan: two-hit word finding + ungapped X-drop extension for one query.
-// The reference (gpu_blastp_kernel.cu:42-320) gives each CUDA thread a whole
-// subject and a private diagonal array in global memory, so every word hit is
-// a random global read-modify-write.  Here a 64-lane wavefront owns a
-// subject; its diagonal table lives in LDS, the query table is staged in LDS
-// once per workgroup, and the lanes work on 64 subject positions at a time:
+// k_scan: two-hit word finding + ungapped X-drop extension of a group of
+// queries against a list of subjects.  The reference (gpu_blastp_kernel.cu:
+// 42-320) gives each CUDA thread a whole subject with a private diagonal
+// array in global memory, so every word hit is a random global read-modify-
+// write and every extension step a dependent global load.  Here a 64-lane
+// wavefront owns one subject at a time:
 //
-//   1. each lane forms the word at its position, tests the presence bits and
-//      looks up its cell (rank into the packed CSR table);
-//   2. a wave prefix sum over the cell sizes compacts the hits, in
-//      (subject offset, query offset) order, into a per-wave LDS ring;
-//   3. the ring is drained 64 hits at a time: each lane takes one hit and
-//      claims its diagonal with an LDS atomic-min on its lane number, so hits
-//      that share a diagonal are applied in scan order while hits on
-//      different diagonals proceed together -- the exact sequential
-//      semantics of the two-hit state machine (gpu_blastp_kernel.cu:194-301);
-//   4. a lane whose hit completes a pair runs the X-drop extension
-//      (s_BlastAaExtendTwoHit/Left/Right, gpu_blastp_kernel.cu:341-549) and
-//      appends a record to its wave's private output segment.
+//   * the subject is staged into a per-wave LDS buffer with 16-byte loads
+//     (long subjects through a sliding window); words and extensions read it
+//     from LDS;
+//   * the workgroup stages the group's packed lookup table, its query
+//     residues and a 28x32 score table into LDS once per group;
+//   * the lanes take 64 subject positions at a time: each forms its word,
+//     tests the presence bits and finds its cell; a DPP prefix sum over the
+//     cell sizes compacts the hits, in (subject offset, query offset) order,
+//     into a per-wave LDS ring;
+//   * the ring is drained 64 hits at a time.  Hits that share a diagonal
+//     slot are applied in scan order by claiming a hashed slot with an LDS
+//     atomic-min on the lane number (gpu_blastp_kernel.cu:194-301 applied
+//     exactly, in parallel across slots);
+//   * a hit that completes a pair does not extend inline: its slot is marked
+//     pending and the extension is queued.  Queued extensions run 64 at a
+//     time (s_BlastAaExtendTwoHit/Left/Right semantics, gpu_blastp_kernel.cu:
+//     341-549); a later hit on a pending slot first flushes the queue, so
+//     every slot still sees its hits strictly in scan order;
+//   * records go to the wave's private output segment; per-(query, subject)
+//     counts, optional counters and per-query ungapped statistics are written
+//     when the subject is done.
 //
-// Records are later moved into subject order by k_tile_sums / k_tile_scan /
-// k_gather (count, scan, copy: no atomics on the output).
+// Records are put in (query, subject, subject offset, query offset) order by
+// the count / scan / gather kernels below (no atomics on the output).
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -30,8 +39,12 @@ namespace bpx {
 
 namespace {
 
-constexpr uint32_t kFlagBit = 0x80000000u;
-constexpr int kMatrixStride = 32;  // LDS matrix rows padded 28 -> 32
+constexpr uint32_t kFlag = 0x80000000u;  // NCBI reset flag: an extension reached past this hit
+constexpr uint32_t kPend = 0x40000000u;  // extension queued, outcome not yet applied
+constexpr uint32_t kVal = 0x3FFFFFFFu;   // last_hit + running diagonal offset
+constexpr int32_t kOffsetLimit = 1 << 28;
+constexpr int kMatrixStride = 32;        // LDS score rows padded 28 -> 32
+constexpr uint32_t kCtlBytes = 16;       // workgroup control words at LDS offset 0
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -43,23 +56,26 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-// Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t rank_below(unsigned long long mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Inclusive prefix sum over the 64 lanes with DPP: Hillis-Steele inside each
+// 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's
+// into row 3 (row_bcast:15), then lane 31's total into rows 2-3
+// (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
 }
 
 struct TwoHit {
@@ -67,30 +83,39 @@ struct TwoHit {
     bool right_extend;
 };
 
-// s_BlastAaExtendTwoHit semantics: best prefix of the second word, left
-// X-drop from one past it, right X-drop only if the left reached the first
-// hit.  Subject bytes come from HBM (L1/L2-resident: the wave is streaming
-// this subject), query bytes and scores from LDS.
-__device__ __forceinline__ TwoHit extend_two_hit(const int8_t* __restrict__ M,
-                                                 const uint8_t* __restrict__ Q, int32_t qlen,
-                                                 const uint8_t* __restrict__ S, int32_t slen,
-                                                 int32_t s_left, int32_t s, int32_t q,
-                                                 int32_t xdrop, int32_t word) {
+// Subject residue p of the wave's staged subject.  RESIDENT: the whole
+// subject is in sbuf; otherwise sbuf is a window [whi - sbuf, whi) and other
+// positions come from HBM.
+template <bool RESIDENT>
+__device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
+                                         int32_t wlo, int32_t slen) {
+    if (RESIDENT || p >= wlo) return sbuf[p & smask];
+    return p < slen ? S[p] : 0u;
+}
+
+// s_BlastAaExtendTwoHit semantics (gpu_blastp_kernel.cu:341-415 with the
+// loops of :435-481 and :503-549): best prefix of the second word, X-drop to
+// the left from one past it, to the right only if the left reached the
+// first hit.  c is the second hit's position in the group's concatenated
+// residues Q; the query occupies Q[qstart, qstart + qlen).
+template <int WORD, bool RESIDENT>
+__device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
+                                                 const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t wlo,
+                                                 int32_t slen, int32_t s_left, int32_t s, int32_t c, int32_t xdrop) {
     int32_t sc = 0, best = 0, rd = 0;
-    for (int32_t i = 0; i < word; ++i) {
-        sc += M[Q[q + i] * kMatrixStride + S[s + i]];
+#pragma unroll
+    for (int32_t i = 0; i < WORD; ++i) {
+        sc += M[Q[c + i] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, s + i, wlo, slen)];
         if (sc > best) {
             best = sc;
             rd = i + 1;
         }
     }
-    const int32_t qr = q + rd, sr = s + rd;
-
-    // left: positions sr-1-k, qr-1-k for k = 0..n
+    const int32_t cr = c + rd, sr = s + rd, qr = cr - qstart;
     const int32_t n = min(sr - 1, qr - 1);
     int32_t score = 0, mx = 0, left_len = 0;
     for (int32_t k = 0; k <= n; ++k) {
-        score += M[Q[qr - 1 - k] * kMatrixStride + S[sr - 1 - k]];
+        score += M[Q[cr - 1 - k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr - 1 - k, wlo, slen)];
         if (score > mx) {
             mx = score;
             left_len = k + 1;
@@ -105,7 +130,7 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* __restrict__ M,
         const int32_t n2 = min(slen - sr, qlen - qr);
         int32_t rscore = mx, rmax = mx, best_k = -1, k = 0;
         for (; k < n2; ++k) {
-            rscore += M[Q[qr + k] * kMatrixStride + S[sr + k]];
+            rscore += M[Q[cr + k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr + k, wlo, slen)];
             if (rscore > rmax) {
                 rmax = rscore;
                 best_k = k;
@@ -123,126 +148,251 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* __restrict__ M,
     return r;
 }
 
-template <int WORD, bool TABLE_IN_LDS>
+// Per-query counters of a wave (MULTI groups), 32 bytes each.
+struct QCount {
+    uint32_t hits, ext, saved, pad;
+    unsigned long long agg_hits;
+    uint32_t agg_nseq, agg_ext, agg_saved, agg_passed;
+};
+
+template <int WORD, bool TAB_LDS, bool MULTI>
 __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const uint32_t wave = tid >> 6;
     const uint32_t nwaves = blockDim.x >> 6;
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(lds);
 
-    // ---- stage the query side into LDS (once per workgroup) ----
-    const uint32_t* table = p.table;
-    if (TABLE_IN_LDS) {
-        uint32_t* t = reinterpret_cast<uint32_t*>(lds + p.lds_table);
-        for (uint32_t i = tid; i < p.table_words; i += blockDim.x) t[i] = p.table[i];
-        table = t;
-    }
     int8_t* M = reinterpret_cast<int8_t*>(lds + p.lds_matrix);
     for (uint32_t i = tid; i < 28 * kMatrixStride; i += blockDim.x) {
         const uint32_t r = i / kMatrixStride, c = i % kMatrixStride;
         M[i] = c < 28 ? p.matrix[r * 28 + c] : (int8_t)-128;
     }
-    uint8_t* Q = lds + p.lds_query;
-    for (int32_t i = (int32_t)tid; i < p.qlen; i += (int32_t)blockDim.x) Q[i] = p.query[i];
+    uint32_t* tab_lds = reinterpret_cast<uint32_t*>(lds + p.lds_table);
+    uint8_t* Q = lds + p.lds_qres;
+    uint8_t* qmap = lds + p.lds_qmap;
+    QueryInfo* qinfo = reinterpret_cast<QueryInfo*>(lds + p.lds_qinfo);
 
-    const uint32_t dlen = (uint32_t)p.diag_mask + 1;
-    uint32_t* diag = reinterpret_cast<uint32_t*>(lds + p.lds_wave + wave * p.wave_stride);
-    uint32_t* claim = diag + dlen;
-    uint32_t* ring = claim + dlen;
+    uint8_t* wbase = lds + p.lds_wave + wave * p.wave_stride;
+    uint8_t* sbuf = wbase;
+    uint32_t* diag = reinterpret_cast<uint32_t*>(wbase + p.sbuf);
+    uint32_t* claim = diag + p.diag_cap;
+    uint32_t* ring = claim + kClaimSlots;
+    uint32_t* t_slot = ring + p.qcap;
+    uint32_t* t_cj = t_slot + kTaskCap;
+    uint32_t* t_s = t_cj + kTaskCap;
+    uint32_t* t_sl = t_s + kTaskCap;
+    QCount* qc = reinterpret_cast<QCount*>(t_sl + kTaskCap);
+    for (uint32_t i = lane; i < p.diag_cap; i += 64) diag[i] = 0;
+    for (uint32_t i = lane; i < kClaimSlots; i += 64) claim[i] = 0xFFFFFFFFu;
+    if (MULTI)
+        for (uint32_t i = lane; i < p.max_nq; i += 64) qc[i] = QCount{};
+
+    const uint32_t smask = p.sbuf - 1;
     const uint32_t qcap = p.qcap, qmask = qcap - 1;
-    for (uint32_t i = lane; i < dlen; i += 64) {
-        diag[i] = 0;
-        claim[i] = 0xFFFFFFFFu;
-    }
-    __syncthreads();
-
-    const uint2* groups = reinterpret_cast<const uint2*>(table);
-    const uint32_t* starts = table + 2 * p.ngroups;
-    const uint16_t* entries = reinterpret_cast<const uint16_t*>(starts + p.npresent + 1);
-    const int32_t mask = p.diag_mask, window = p.window, xdrop = p.xdrop, cutoff = p.cutoff;
-
-    const uint32_t wave_global = blockIdx.x * nwaves + wave;
-    const uint32_t seg_base = wave_global * p.seg_cap;
-    uint32_t seg_fill = 0;          // wave-uniform
-    int32_t doff = window;          // NCBI diagonal offset, wave-uniform
+    const int32_t window = p.window;
+    const uint32_t seg_base = (p.seg_wave_base + blockIdx.x * nwaves + wave) * p.seg_cap;
+    uint32_t seg_fill = 0;      // wave-uniform
+    int32_t doff = window;      // running NCBI diagonal offset, wave-uniform
+    uint32_t cur_group = 0xFFFFFFFFu;
+    GroupInfo g{};
 
     for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(p.work_counter, p.grab);
-        item = uniform(__shfl(item, 0, 64));
-        if (item >= p.norder) break;
-        const uint32_t item_end = min(item + p.grab, p.norder);
-        for (; item < item_end; ++item) {
-            const uint32_t sid = p.order[item];
+        __syncthreads();
+        if (tid == 0) {
+            ctl[0] = atomicAdd(p.work_counter, 1u);
+            ctl[1] = 0;
+        }
+        __syncthreads();
+        const uint32_t item = ctl[0];
+        if (item >= p.nitems) break;
+        const WorkItem w = p.items[item];
+        if (w.group != cur_group) {
+            g = p.groups[w.group];
+            if (TAB_LDS)
+                for (uint32_t i = tid; i < g.table_words; i += blockDim.x) tab_lds[i] = p.tables[g.table_word + i];
+            for (uint32_t i = tid; i < g.qres_len; i += blockDim.x) Q[i] = p.qres[g.qres_off + i];
+            if (MULTI)
+                for (uint32_t i = tid; i < g.qres_len; i += blockDim.x) {
+                    uint32_t j = 0;
+                    while (j + 1 < g.nq && p.qinfo[g.qinfo_first + j + 1].qstart <= i) ++j;
+                    qmap[i] = (uint8_t)j;
+                }
+            for (uint32_t i = tid; i < g.nq; i += blockDim.x) qinfo[i] = p.qinfo[g.qinfo_first + i];
+            cur_group = w.group;
+            __syncthreads();
+        }
+        const uint32_t* table = TAB_LDS ? tab_lds : p.tables + g.table_word;
+        const uint2* pvr = reinterpret_cast<const uint2*>(table);
+        const uint32_t* starts = table + 2 * p.ngroups_pv;
+        const uint16_t* entries = reinterpret_cast<const uint16_t*>(starts + g.npresent + 1);
+        const QueryInfo q0 = qinfo[0];
+        uint32_t a_nseq = 0, a_ext = 0, a_saved = 0, a_passed = 0;  // single-query aggregates
+        unsigned long long a_hits = 0;
+
+        for (;;) {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&ctl[1], 1u);
+            k = uniform(__shfl(k, 0, 64));
+            if (k >= w.count) break;
+            const uint32_t sid = p.order[w.first + k];
+            const uint32_t slot = sid - p.first;
             const uint8_t* __restrict__ S = p.residues + p.soff[sid];
             const int32_t slen = (int32_t)p.slen[sid];
-            const int32_t last = slen - WORD;  // last word start
-            uint32_t hits = 0, extended = 0, saved = 0;
+            const int32_t send = (slen + 15) & ~15;
+            const bool resident = slen <= (int32_t)p.sbuf;
+
+            // ---- stage the subject (or its first window) into LDS ----
+            auto fill = [&](int32_t from, int32_t to) {
+                for (int32_t x = from + 16 * (int32_t)lane; x < to; x += 1024)
+                    *reinterpret_cast<uint4*>(sbuf + (x & smask)) = *reinterpret_cast<const uint4*>(S + x);
+            };
+            int32_t whi = resident ? send : min(send, (int32_t)p.sbuf);
+            fill(0, whi);
+            wave_sync();
+
+            uint32_t hits = 0, extended = 0, saved = 0;   // single-query counters
             const uint32_t rec_start = seg_fill;
-            uint32_t head = 0, count = 0;  // ring state, wave-uniform
-            int32_t qbase = 0;             // subject offset of ring entries' base
+            uint32_t head = 0, count = 0, ntasks = 0;     // wave-uniform
+            int32_t qbase = 0;
+
+            // Runs the queued extensions, one per lane, and applies their
+            // outcome to the diagonal table.
+            auto flush_tasks = [&]() {
+                const uint32_t nt = ntasks;
+                if (nt == 0) return;
+                wave_sync();
+                const bool mine = lane < nt;
+                bool rec = false;
+                TwoHit x{};
+                uint32_t j = 0, dslot = 0;
+                int32_t s = 0, c = 0;
+                if (mine) {
+                    dslot = t_slot[lane];
+                    const uint32_t cj = t_cj[lane];
+                    c = (int32_t)(cj >> 16);
+                    j = cj & 0xFFFFu;
+                    s = (int32_t)t_s[lane];
+                    const int32_t sl = (int32_t)t_sl[lane];
+                    const QueryInfo qi = MULTI ? qinfo[j] : q0;
+                    const int32_t wlo = whi - (int32_t)p.sbuf;
+                    if (resident)
+                        x = extend_two_hit<WORD, true>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
+                                                       wlo, slen, sl, s, c, qi.xdrop);
+                    else
+                        x = extend_two_hit<WORD, false>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
+                                                        wlo, slen, sl, s, c, qi.xdrop);
+                    rec = x.score >= qi.cutoff;
+                    diag[dslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
+                                                 : (uint32_t)(s + doff);
+                    if (MULTI) {
+                        atomicAdd(&qc[j].ext, 1u);
+                        if (rec) atomicAdd(&qc[j].saved, 1u);
+                    }
+                    c -= (int32_t)qi.qstart;  // query-local offset of the second hit
+                }
+                const unsigned long long rb = __ballot(rec);
+                if (!MULTI) {
+                    extended += nt;
+                    saved += (uint32_t)__popcll(rb);
+                }
+                if (rb) {
+                    const uint32_t at = seg_fill + rank_below(rb);
+                    if (rec) {
+                        if (at < p.seg_cap) {
+                            SegRecord r;
+                            r.h.hsp_q = x.hsp_q;
+                            r.h.hsp_s = x.hsp_s;
+                            r.h.query_offset = c;
+                            r.h.subject_offset = s;
+                            r.h.hsp_len = x.len;
+                            r.h.score = x.score;
+                            r.j = j;
+                            r.pad = 0;
+                            p.seg[seg_base + at] = r;
+                        } else {
+                            atomicOr(p.overflow, 1u);
+                        }
+                    }
+                    seg_fill += (uint32_t)__popcll(rb);
+                }
+                ntasks = 0;
+                wave_sync();
+            };
 
             // Applies the oldest n (<= 64) ring entries to the diagonal table.
             auto drain_batch = [&](uint32_t n) {
                 bool active = lane < n;
-                uint32_t e = 0;
-                if (active) e = ring[(head + lane) & qmask];
-                const int32_t q = (int32_t)(e >> 16);
+                const uint32_t e = active ? ring[(head + lane) & qmask] : 0u;
+                const int32_t c = (int32_t)(e >> 16);
                 const int32_t s = qbase + (int32_t)(e & 0xFFFFu);
-                const uint32_t d = (uint32_t)((q - s) & mask);
+                uint32_t j = 0, dbase = 0, dmask = q0.dmask;
+                int32_t qstart = (int32_t)q0.qstart;
+                if (MULTI) {
+                    j = active ? qmap[c] : 0u;
+                    const QueryInfo qi = qinfo[j];
+                    dbase = qi.dbase;
+                    dmask = qi.dmask;
+                    qstart = (int32_t)qi.qstart;
+                    if (active) atomicAdd(&qc[j].hits, 1u);
+                } else {
+                    dbase = q0.dbase;
+                }
+                const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
+                const uint32_t h = dslot & (kClaimSlots - 1);
                 while (__ballot(active)) {
-                    if (active) atomicMin(&claim[d], lane);
+                    if (active) atomicMin(&claim[h], lane);
                     wave_sync();
-                    const bool win = active && claim[d] == lane;
+                    const bool win = active && claim[h] == lane;
                     wave_sync();
-                    bool rec = false, ext = false;
-                    TwoHit x;
+                    bool conflict = false, task = false, write = false;
+                    uint32_t ns = 0;
+                    int32_t sl = 0;
                     if (win) {
-                        claim[d] = 0xFFFFFFFFu;
-                        const uint32_t st = diag[d];
-                        const int32_t lastv = (int32_t)(st & ~kFlagBit);
-                        uint32_t ns = st;
-                        if (st & kFlagBit) {
-                            if (s + doff >= lastv) ns = (uint32_t)(s + doff);
+                        claim[h] = 0xFFFFFFFFu;
+                        const uint32_t st = diag[dslot];
+                        if (st & kPend) {
+                            conflict = true;
                         } else {
-                            const int32_t lh = lastv - doff;
-                            const int32_t diff = s - lh;
-                            if (diff >= window) {
-                                ns = (uint32_t)(s + doff);
-                            } else if (diff >= WORD) {
-                                ext = true;
-                                x = extend_two_hit(M, Q, p.qlen, S, slen, lh + WORD, s, q, xdrop, WORD);
-                                rec = x.score >= cutoff;
-                                ns = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlagBit)
-                                                    : (uint32_t)(s + doff);
-                            }
-                        }
-                        diag[d] = ns;
-                        active = false;
-                    }
-                    extended += (uint32_t)__popcll(__ballot(ext));
-                    const unsigned long long rb = __ballot(rec);
-                    if (rb) {
-                        const uint32_t slot = seg_fill + rank_below(rb);
-                        if (rec) {
-                            if (slot < p.seg_cap) {
-                                bpx_init_hsp h;
-                                h.hsp_q = x.hsp_q;
-                                h.hsp_s = x.hsp_s;
-                                h.query_offset = q;
-                                h.subject_offset = s;
-                                h.hsp_len = x.len;
-                                h.score = x.score;
-                                p.seg[seg_base + slot] = h;
+                            active = false;
+                            const int32_t v = (int32_t)(st & kVal);
+                            if (st & kFlag) {
+                                if (s + doff >= v) {
+                                    ns = (uint32_t)(s + doff);
+                                    write = true;
+                                }
                             } else {
-                                atomicOr(p.overflow, 1u);
+                                const int32_t lh = v - doff;
+                                const int32_t diff = s - lh;
+                                if (diff >= window) {
+                                    ns = (uint32_t)(s + doff);
+                                    write = true;
+                                } else if (diff >= WORD) {
+                                    task = true;
+                                    sl = lh + WORD;
+                                    ns = kPend;
+                                    write = true;
+                                }
                             }
                         }
-                        seg_fill += (uint32_t)__popcll(rb);
-                        saved += (uint32_t)__popcll(rb);
                     }
+                    const unsigned long long tb = __ballot(task);
+                    if (tb) {
+                        const uint32_t nt = (uint32_t)__popcll(tb);
+                        if (ntasks + nt > kTaskCap) flush_tasks();
+                        if (task) {
+                            const uint32_t idx = ntasks + rank_below(tb);
+                            t_slot[idx] = dslot;
+                            t_cj[idx] = ((uint32_t)c << 16) | j;
+                            t_s[idx] = (uint32_t)s;
+                            t_sl[idx] = (uint32_t)sl;
+                        }
+                        ntasks += nt;
+                    }
+                    if (write) diag[dslot] = ns;
                     wave_sync();
+                    if (__ballot(conflict)) flush_tasks();
                 }
                 head = (head + n) & qmask;
                 count -= n;
@@ -251,7 +401,14 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 while (count > 0) drain_batch(min(count, 64u));
             };
 
+            const int32_t last = slen - WORD;
             for (int32_t pos = 0; pos <= last;) {
+                if (!resident && pos + 128 > whi && whi < send) {
+                    const int32_t nhi = min(send, whi + (int32_t)(p.sbuf >> 1));
+                    fill(whi, nhi);
+                    whi = nhi;
+                    wave_sync();
+                }
                 if (count == 0) {
                     qbase = pos;
                 } else if (pos + 64 - qbase >= 65536) {
@@ -261,21 +418,20 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 const int32_t s = pos + (int32_t)lane;
                 uint32_t cnt = 0, start = 0;
                 if (s <= last) {
-                    uint32_t w = S[s];
+                    uint32_t wd = sbuf[s & smask];
 #pragma unroll
-                    for (int i = 1; i < WORD; ++i) w = (w << 5) | S[s + i];
-                    const uint2 g = groups[w >> 5];
-                    const uint32_t bit = 1u << (w & 31u);
-                    if (g.x & bit) {
-                        const uint32_t r = g.y + (uint32_t)__popc(g.x & (bit - 1u));
+                    for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
+                    const uint2 gv = pvr[wd >> 5];
+                    const uint32_t bit = 1u << (wd & 31u);
+                    if (gv.x & bit) {
+                        const uint32_t r = gv.y + (uint32_t)__popc(gv.x & (bit - 1u));
                         start = starts[r];
                         cnt = starts[r + 1] - start;
                     }
                 }
-                const uint32_t incl = wave_inclusive_sum(cnt, lane);
+                const uint32_t incl = wave_inclusive_sum(cnt);
                 const bool fits = incl <= qcap - count;
-                const unsigned long long fb = __ballot(fits);
-                const uint32_t nfit = (uint32_t)__popcll(fb);
+                const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
                 if (nfit == 0) {
                     if (count == 0) {  // a cell larger than the ring: host sizing error
                         if (lane == 0) atomicOr(p.overflow, 2u);
@@ -287,8 +443,7 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 if (fits && cnt) {
                     const uint32_t at = head + count + (incl - cnt);
                     const uint32_t sv = (uint32_t)(s - qbase);
-                    for (uint32_t j = 0; j < cnt; ++j)
-                        ring[(at + j) & qmask] = ((uint32_t)entries[start + j] << 16) | sv;
+                    for (uint32_t t = 0; t < cnt; ++t) ring[(at + t) & qmask] = ((uint32_t)entries[start + t] << 16) | sv;
                 }
                 const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
                 count += added;
@@ -298,35 +453,101 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 while (count >= 64) drain_batch(64);
             }
             drain_all();
+            flush_tasks();
 
-            if (lane == 0) {
-                const uint32_t slot = sid - p.first;
-                p.stats[3 * (size_t)slot + 0] = (int32_t)hits;
-                p.stats[3 * (size_t)slot + 1] = (int32_t)saved;
-                p.stats[3 * (size_t)slot + 2] = (int32_t)extended;
-                p.rec_loc[slot] = seg_base + rec_start;
+            // ---- per-subject outputs ----
+            const uint32_t gslot = w.group * p.count + slot;
+            if (!MULTI) {
+                if (lane == 0) {
+                    const size_t i = (size_t)q0.qid * p.count + slot;
+                    p.cnt[i] = saved;
+                    if (p.stats) {
+                        p.stats[3 * i + 0] = (int32_t)hits;
+                        p.stats[3 * i + 1] = (int32_t)saved;
+                        p.stats[3 * i + 2] = (int32_t)extended;
+                    }
+                    p.run_loc[gslot] = seg_base + rec_start;
+                    p.run_len[gslot] = seg_fill - rec_start;
+                }
+                if (hits > 0) {  // Blast_UngappedStatsUpdate skips subjects without hits
+                    a_hits += hits;
+                    a_nseq += 1;
+                    a_ext += extended;
+                    a_saved += saved;
+                    a_passed += saved > 0;
+                }
+            } else {
+                wave_sync();
+                for (uint32_t jj = lane; jj < g.nq; jj += 64) {
+                    QCount& cq = qc[jj];
+                    const size_t i = (size_t)qinfo[jj].qid * p.count + slot;
+                    p.cnt[i] = cq.saved;
+                    if (p.stats) {
+                        p.stats[3 * i + 0] = (int32_t)cq.hits;
+                        p.stats[3 * i + 1] = (int32_t)cq.saved;
+                        p.stats[3 * i + 2] = (int32_t)cq.ext;
+                    }
+                    if (cq.hits > 0) {
+                        cq.agg_hits += cq.hits;
+                        cq.agg_nseq += 1;
+                        cq.agg_ext += cq.ext;
+                        cq.agg_saved += cq.saved;
+                        cq.agg_passed += cq.saved > 0;
+                    }
+                    cq.hits = cq.ext = cq.saved = 0;
+                }
+                if (lane == 0) {
+                    p.run_loc[gslot] = seg_base + rec_start;
+                    p.run_len[gslot] = seg_fill - rec_start;
+                }
+                wave_sync();
             }
-            // Blast_ExtendWordExit: move to a fresh diagonal range, clearing
-            // the table before the 31-bit field can run out.
-            if (doff >= 0x7FFFFFFF / 4) {
-                for (uint32_t i = lane; i < dlen; i += 64) diag[i] = 0;
+            // Blast_ExtendWordExit: a fresh diagonal range for the next
+            // subject; clear the table before the offset field runs out.
+            if (doff >= kOffsetLimit) {
+                for (uint32_t i = lane; i < p.diag_cap; i += 64) diag[i] = 0;
                 doff = window;
                 wave_sync();
             } else {
                 doff += slen + window;
             }
         }
+
+        // ---- per-query ungapped statistics of this work item ----
+        if (!MULTI) {
+            if (lane == 0 && a_nseq) {
+                unsigned long long* u = p.ustats + 5 * (size_t)q0.qid;
+                atomicAdd(&u[0], a_hits);
+                atomicAdd(&u[1], (unsigned long long)a_nseq);
+                atomicAdd(&u[2], (unsigned long long)a_ext);
+                atomicAdd(&u[3], (unsigned long long)a_saved);
+                atomicAdd(&u[4], (unsigned long long)a_passed);
+            }
+        } else {
+            for (uint32_t jj = lane; jj < g.nq; jj += 64) {
+                QCount& cq = qc[jj];
+                if (cq.agg_nseq) {
+                    unsigned long long* u = p.ustats + 5 * (size_t)qinfo[jj].qid;
+                    atomicAdd(&u[0], cq.agg_hits);
+                    atomicAdd(&u[1], (unsigned long long)cq.agg_nseq);
+                    atomicAdd(&u[2], (unsigned long long)cq.agg_ext);
+                    atomicAdd(&u[3], (unsigned long long)cq.agg_saved);
+                    atomicAdd(&u[4], (unsigned long long)cq.agg_passed);
+                }
+                cq = QCount{};
+            }
+        }
     }
 }
 
-// ---- compaction -------------------------------------------------------------
+// ---- compaction ----------------------------------------------------------------
 
 constexpr uint32_t kCompactThreads = 256;
-constexpr uint32_t kPer = kTile / kCompactThreads;  // 4 subjects per thread
+constexpr uint32_t kPer = kTile / kCompactThreads;  // 4 counts per thread
 
 __device__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* sh, uint32_t* total) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const uint32_t incl = wave_inclusive_sum(v, lane);
+    const uint32_t incl = wave_inclusive_sum(v);
     if (lane == 63) sh[wave] = incl;
     __syncthreads();
     uint32_t base = 0, all = 0;
@@ -339,43 +560,17 @@ __device__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* sh, uint32_t* tota
     return base + incl - v;
 }
 
-// Per tile: sum of saved counts, plus the query-level ungapped statistics
-// Blast_UngappedStatsUpdate accumulates (blast_engine.c:553-556).
 __global__ __launch_bounds__(kCompactThreads) void k_tile_sums(CompactParams p) {
     __shared__ uint32_t sh[kCompactThreads / 64];
-    __shared__ unsigned long long acc[5];
-    if (threadIdx.x < 5) acc[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t sum = 0;
-    unsigned long long h = 0, nh = 0, ex = 0, sv = 0, np = 0;
-    for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t i = base + k;
-        if (i < p.count) {
-            const int32_t* st = p.stats + 3 * (size_t)i;
-            sum += (uint32_t)st[1];
-            if (st[0] > 0) {  // Blast_UngappedStatsUpdate ignores subjects without hits
-                h += (uint32_t)st[0];
-                nh += 1;
-                ex += (uint32_t)st[2];
-                sv += (uint32_t)st[1];
-                np += st[1] > 0;
-            }
-        }
-    }
+    for (uint32_t k = 0; k < kPer; ++k)
+        if (base + k < p.n) sum += p.cnt[base + k];
     uint32_t tot;
     block_exclusive_sum(sum, sh, &tot);
     if (threadIdx.x == 0) p.tile_sum[blockIdx.x] = tot;
-    atomicAdd(&acc[0], h);
-    atomicAdd(&acc[1], nh);
-    atomicAdd(&acc[2], ex);
-    atomicAdd(&acc[3], sv);
-    atomicAdd(&acc[4], np);
-    __syncthreads();
-    if (threadIdx.x < 5 && acc[threadIdx.x]) atomicAdd(&p.ustats[threadIdx.x], acc[threadIdx.x]);
 }
 
-// Exclusive scan of the tile sums in one workgroup (tile offsets in place).
 __global__ __launch_bounds__(1024) void k_tile_scan(CompactParams p, uint32_t ntiles) {
     __shared__ uint32_t sh[16];
     uint32_t carry = 0;
@@ -388,112 +583,141 @@ __global__ __launch_bounds__(1024) void k_tile_scan(CompactParams p, uint32_t nt
         carry += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        p.rec_off[p.count] = carry;
-        p.ustats[5] = carry;
-    }
+    if (threadIdx.x == 0) *p.total = carry;
 }
 
-// Per subject: CSR offset, then copy its records from the wave segment in
-// (subject_offset, query_offset) order -- the sequential emission order.
-__global__ __launch_bounds__(kCompactThreads) void k_gather(CompactParams p) {
+// counts -> exclusive offsets, in place
+__global__ __launch_bounds__(kCompactThreads) void k_offsets(CompactParams p) {
     __shared__ uint32_t sh[kCompactThreads / 64];
-    if (*p.overflow) return;  // segments incomplete; the host rescans
-    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t c[kPer], sum = 0;
     for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t i = base + k;
-        c[k] = i < p.count ? (uint32_t)p.stats[3 * (size_t)i + 1] : 0u;
+        c[k] = base + k < p.n ? p.cnt[base + k] : 0u;
         sum += c[k];
     }
     uint32_t tot;
     uint32_t off = p.tile_sum[blockIdx.x] + block_exclusive_sum(sum, sh, &tot);
-    for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t i = base + k;
-        if (i >= p.count) break;
-        p.rec_off[i] = off;
-        const bpx_init_hsp* src = p.seg + p.rec_loc[i];
-        bpx_init_hsp* dst = p.records + off;
-        for (uint32_t j = 0; j < c[k]; ++j) {
-            const bpx_init_hsp h = src[j];
-            uint32_t m = j;  // insertion sort on (subject_offset, query_offset)
-            while (m > 0) {
-                const bpx_init_hsp g = dst[m - 1];
-                if (g.subject_offset < h.subject_offset ||
-                    (g.subject_offset == h.subject_offset && g.query_offset < h.query_offset))
-                    break;
-                dst[m] = g;
-                --m;
-            }
-            dst[m] = h;
-            p.subjects[off + j] = p.first + i;
+    for (uint32_t k = 0; k < kPer; ++k)
+        if (base + k < p.n) {
+            p.cnt[base + k] = off;
+            off += c[k];
         }
-        off += c[k];
+}
+
+// One (group, subject) run of records: order it by (query, subject offset,
+// query offset) -- the sequential emission order within each query -- and
+// move each record to its CSR position.
+__global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p) {
+    if (*p.overflow) return;  // segments incomplete; the host rescans
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)p.ngroups * p.count) return;
+    const uint32_t len = p.run_len[i];
+    if (len == 0) return;
+    const uint32_t g = (uint32_t)(i / p.count), slot = (uint32_t)(i % p.count);
+    SegRecord* run = p.seg + p.run_loc[i];
+    for (uint32_t a = 1; a < len; ++a) {
+        const SegRecord r = run[a];
+        uint32_t m = a;
+        while (m > 0) {
+            const SegRecord& o = run[m - 1];
+            const bool before = o.j < r.j || (o.j == r.j && (o.h.subject_offset < r.h.subject_offset ||
+                                                              (o.h.subject_offset == r.h.subject_offset &&
+                                                               o.h.query_offset < r.h.query_offset)));
+            if (before) break;
+            run[m] = run[m - 1];
+            --m;
+        }
+        run[m] = r;
+    }
+    const GroupInfo gi = p.groups[g];
+    uint32_t cur_j = 0xFFFFFFFFu, dst = 0, qid = 0;
+    for (uint32_t a = 0; a < len; ++a) {
+        const SegRecord r = run[a];
+        if (r.j != cur_j) {
+            cur_j = r.j;
+            qid = p.qinfo[gi.qinfo_first + r.j].qid;
+            dst = p.cnt[(uint64_t)qid * p.count + slot];
+        }
+        p.records[dst] = r.h;
+        p.subjects[dst] = p.first + slot;
+        p.queries[dst] = qid;
+        ++dst;
     }
 }
 
 __global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = value;
 }
 
 }  // namespace
 
-LdsPlan plan_lds(uint32_t table_bytes, int32_t qlen, int32_t diag_len, uint32_t qcap, int waves,
-                 uint32_t lds_limit, bool allow_table_in_lds) {
+LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap, uint32_t qcap,
+                 uint32_t sbuf, int waves, bool table_in_lds) {
     auto up16 = [](uint32_t v) { return (v + 15u) & ~15u; };
     LdsPlan pl{};
-    pl.wave_stride = up16((uint32_t)diag_len * 8u + qcap * 4u);
-    const uint32_t fixed = up16(28 * kMatrixStride) + up16((uint32_t)qlen) + pl.wave_stride * (uint32_t)waves;
-    pl.table_in_lds = allow_table_in_lds && up16(table_bytes) + fixed <= lds_limit;
-    uint32_t at = 0;
+    pl.max_nq = max_nq < 1 ? 1 : max_nq;
+    pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * 4) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 +
+                     (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
+    uint32_t at = kCtlBytes;
     pl.table = at;
-    if (pl.table_in_lds) at += up16(table_bytes);
+    pl.table_cap_words = table_in_lds ? up16(max_table_bytes) / 4 : 0u;
+    at += pl.table_cap_words * 4;
+    pl.qres = at;
+    at += pl.qres_cap;
+    pl.qmap = at;
+    if (pl.max_nq > 1) at += pl.qres_cap;
+    pl.qinfo = at;
+    at += pl.max_nq * (uint32_t)sizeof(QueryInfo);
     pl.matrix = at;
     at += up16(28 * kMatrixStride);
-    pl.query = at;
-    at += up16((uint32_t)qlen);
     pl.wave = at;
     at += pl.wave_stride * (uint32_t)waves;
     pl.total = at;
     return pl;
 }
 
-hipError_t launch_scan(const ScanParams& p, bool table_in_lds, int waves, int groups,
+hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, int waves, int groups,
                        hipStream_t stream) {
     const dim3 grid((unsigned)groups), block((unsigned)(64 * waves));
     void (*fn)(ScanParams) = nullptr;
+#define BPX_PICK(W)                                                                                   \
+    fn = table_in_lds ? (multi ? k_scan<W, true, true> : k_scan<W, true, false>)                      \
+                      : (multi ? k_scan<W, false, true> : k_scan<W, false, false>)
     if (p.word == 3)
-        fn = table_in_lds ? k_scan<3, true> : k_scan<3, false>;
+        BPX_PICK(3);
     else if (p.word == 2)
-        fn = table_in_lds ? k_scan<2, true> : k_scan<2, false>;
+        BPX_PICK(2);
     else
         return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_total);
+#undef BPX_PICK
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)p.lds_total);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, grid, block, p.lds_total, stream, p);
     return hipGetLastError();
 }
 
 hipError_t launch_compact(const CompactParams& p, hipStream_t stream) {
-    const uint32_t ntiles = (p.count + kTile - 1) / kTile;
-    if (ntiles == 0) {
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, p, 0u);
-        return hipGetLastError();
+    const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+    if (ntiles > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    if (ntiles) hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(kCompactThreads), 0, stream, p);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, p, (uint32_t)ntiles);
+    if (ntiles) hipLaunchKernelGGL(k_offsets, dim3((unsigned)ntiles), dim3(kCompactThreads), 0, stream, p);
+    const uint64_t runs = (uint64_t)p.ngroups * p.count;
+    if (runs) {
+        const uint64_t blocks = (runs + kCompactThreads - 1) / kCompactThreads;
+        hipLaunchKernelGGL(k_gather_runs, dim3((unsigned)blocks), dim3(kCompactThreads), 0, stream, p);
     }
-    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kCompactThreads), 0, stream, p);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, p, ntiles);
-    hipLaunchKernelGGL(k_gather, dim3(ntiles), dim3(kCompactThreads), 0, stream, p);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0,
-                       stream, dst, value, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream, dst, value,
+                       n);
     return hipGetLastError();
 }
 

@@ -1,6 +1,5 @@
 // kernels.hpp -- launch interface between the C-ABI runtime and the gfx950
-// word-finder kernels.  Host-visible structs only; no HIP types leak into
-// the public header.
+// word-finder kernels.  Host-visible structs only.
 #pragma once
 
 #include <cstdint>
@@ -11,62 +10,120 @@
 
 namespace bpx {
 
-// Word-finder launch for one query over a list of subjects.
+// One query of a scan group (device array, 32 B).  A group is a set of
+// queries scanned together against each subject through one combined lookup
+// table; a single-query scan is a group of one.
+struct QueryInfo {
+    uint32_t qstart;   // offset of the query in its group's concatenated residues
+    uint32_t qlen;
+    uint32_t dbase;    // first slot of the query's diagonal table
+    uint32_t dmask;    // diag_len - 1
+    int32_t xdrop;
+    int32_t cutoff;
+    uint32_t qid;      // index of the query in the scan (row of the count arrays)
+    uint32_t pad;
+};
+
+struct GroupInfo {
+    uint32_t table_word;   // offset (in u32 words) of the group's packed table
+    uint32_t table_words;
+    uint32_t npresent;     // present words (rank space)
+    uint32_t qres_off;     // byte offset of the group's concatenated residues
+    uint32_t qres_len;
+    uint32_t qinfo_first;  // first QueryInfo of the group
+    uint32_t nq;           // queries in the group
+    uint32_t diag_slots;   // sum of the members' diagonal-table lengths
+};
+
+// A unit of dynamic work: one group against a run of the subject order.
+struct WorkItem {
+    uint32_t group;
+    uint32_t first;   // index into the order list
+    uint32_t count;
+    uint32_t pad;
+};
+
+// Record as staged in a wavefront's output segment: the HSP plus the index
+// of its query within the group.
+struct SegRecord {
+    bpx_init_hsp h;
+    uint32_t j;
+    uint32_t pad;
+};
+
 struct ScanParams {
     // database arena
     const uint8_t* residues;
-    const uint64_t* soff;       // per subject: arena offset
-    const uint32_t* slen;       // per subject: length
-    const uint32_t* order;      // work list of absolute subject ids
-    uint32_t norder;
-    uint32_t first;             // subject id of stats/rec_loc slot 0
-    // query table (packed: groups | starts | entries)
-    const uint32_t* table;
-    uint32_t table_words;
-    uint32_t ngroups;
-    uint32_t npresent;
-    const uint8_t* query;
-    int32_t qlen;
+    const uint64_t* soff;
+    const uint32_t* slen;
+    const uint32_t* order;      // subject ids, processing order
+    uint32_t first;             // subject id of slot 0
+    uint32_t count;             // subject slots
+    // queries
+    const GroupInfo* groups;
+    const QueryInfo* qinfo;
+    const uint32_t* tables;
+    const uint8_t* qres;
     const int8_t* matrix;       // 28 x 28, row = query residue
-    int32_t word, window, xdrop, cutoff, diag_mask;
-    // scheduling
+    uint32_t ngroups_pv;        // presence-vector words (nwords / 32), same for all groups
+    int32_t word, window;
+    // work
+    const WorkItem* items;
+    uint32_t nitems;
     uint32_t* work_counter;
-    uint32_t grab;
     // outputs
-    int32_t* stats;             // 3 per subject slot
-    uint32_t* rec_loc;          // per subject slot: first record in seg
-    bpx_init_hsp* seg;          // per-wave record segments
+    uint32_t* cnt;              // saved records per (qid, slot): qid * count + slot
+    int32_t* stats;             // optional: 3 per (qid, slot)
+    uint32_t* run_loc;          // per (group, slot): first segment record of the run
+    uint32_t* run_len;          // per (group, slot): records in the run
+    unsigned long long* ustats; // 5 per qid (bpx_ungapped_stats order)
+    SegRecord* seg;
     uint32_t seg_cap;           // records per wave segment
-    uint32_t* overflow;         // bit 0: a segment filled; bit 1: queue too small
-    // LDS layout (bytes)
-    uint32_t lds_table, lds_matrix, lds_query, lds_wave, wave_stride, qcap;
+    uint32_t seg_wave_base;     // first wave segment of this launch
+    uint32_t* overflow;         // bit 0: a segment filled; bit 1: ring too small
+    // LDS layout (bytes) and shape
+    uint32_t lds_table, lds_qres, lds_qmap, lds_qinfo, lds_matrix, lds_wave, wave_stride;
     uint32_t lds_total;
+    uint32_t table_cap_words;   // LDS table region (words); 0 = tables read from HBM
+    uint32_t qres_cap;          // LDS bytes for concatenated residues / query map
+    uint32_t max_nq;            // LDS slots for QueryInfo / per-query counters
+    uint32_t diag_cap;          // diagonal slots per wave
+    uint32_t qcap;              // hit ring entries per wave (power of two)
+    uint32_t sbuf;              // subject buffer bytes per wave (power of two)
 };
 
-// LDS bytes the word finder needs per workgroup; table_in_lds selects
-// whether the query table is staged.
 struct LdsPlan {
-    uint32_t table, matrix, query, wave, wave_stride, total;
-    bool table_in_lds;
+    uint32_t table, qres, qmap, qinfo, matrix, wave, wave_stride, total;
+    uint32_t table_cap_words, qres_cap, max_nq;
 };
-LdsPlan plan_lds(uint32_t table_bytes, int32_t qlen, int32_t diag_len, uint32_t qcap,
-                 int waves, uint32_t lds_limit, bool allow_table_in_lds);
 
-hipError_t launch_scan(const ScanParams& p, bool table_in_lds, int waves, int groups,
+constexpr uint32_t kClaimSlots = 256;  // hashed claim table entries per wave
+constexpr uint32_t kTaskCap = 64;      // deferred extensions per wave
+
+LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
+                 uint32_t qcap, uint32_t sbuf, int waves, bool table_in_lds);
+
+hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, int waves, int groups,
                        hipStream_t stream);
 
-// Compaction: per-subject saved counts -> CSR offsets -> ordered records.
+// Compaction: counts -> offsets (in place) -> records in (query, subject,
+// subject_offset, query_offset) order.
 struct CompactParams {
-    const int32_t* stats;       // 3 per subject slot
-    const uint32_t* rec_loc;
-    const bpx_init_hsp* seg;
+    uint32_t* cnt;              // nq * count saved counts; overwritten by offsets
+    uint64_t n;                 // nq * count
+    uint32_t* tile_sum;
+    const uint32_t* run_loc;
+    const uint32_t* run_len;
+    const GroupInfo* groups;
+    const QueryInfo* qinfo;
+    uint32_t ngroups;
     uint32_t count;             // subject slots
-    uint32_t first;             // absolute id of slot 0
-    uint32_t* tile_sum;         // ceil(count / kTile)
-    uint32_t* rec_off;          // count + 1
-    bpx_init_hsp* records;      // capacity >= total
-    uint32_t* subjects;         // capacity >= total
-    unsigned long long* ustats; // 5 sums (bpx_ungapped_stats order) + total records
+    uint32_t first;
+    SegRecord* seg;
+    bpx_init_hsp* records;
+    uint32_t* subjects;
+    uint32_t* queries;
+    unsigned long long* total;  // total records
     const uint32_t* overflow;
 };
 constexpr uint32_t kTile = 1024;

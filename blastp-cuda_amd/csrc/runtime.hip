@@ -1,12 +1,12 @@
 // runtime.hip -- the C ABI of include/bpx.h: contexts, the resident residue
-// arena, query tables, scans and their results.  One HIP stream per
-// context; every device buffer is owned by a handle.
+// arena, query sets (one query or a batch packed into scan groups), scans and
+// their results.  One HIP stream per context; every device buffer is owned
+// by a handle.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
-#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -35,24 +35,26 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(BPX_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                               \
+#define HIP_TRY(expr)                                                                                         \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail(BPX_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
     } while (0)
 
 constexpr uint32_t kArenaAlign = 16;   // subject starts in the arena
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
+constexpr int kMaxClasses = 4;         // (table in LDS?) x (multi-query groups?)
+constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 
-// Size-keyed cache of device allocations, so repeated scans do not call
-// hipMalloc/hipFree.
-struct DevicePool {
-    explicit DevicePool(bool host = false) : host_(host) {}
+// Size-keyed cache of device (or pinned host) allocations, so repeated scans
+// do not call hipMalloc/hipFree.
+struct Pool {
+    explicit Pool(bool host = false) : host_(host) {}
     bool host_;
     std::multimap<size_t, void*> free_;
-    ~DevicePool() {
+    std::map<void*, size_t> sizes_;
+    ~Pool() {
         for (auto& kv : free_) release(kv.second);
     }
     void release(void* p) {
@@ -65,7 +67,7 @@ struct DevicePool {
         return host_ ? hipHostMalloc(p, bytes, hipHostMallocDefault) : hipMalloc(p, bytes);
     }
     void* get(size_t bytes) {
-        bytes = (bytes + 255) & ~size_t(255);
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
         auto it = free_.lower_bound(bytes);
         if (it != free_.end() && it->first <= 2 * bytes) {
             void* p = it->second;
@@ -74,8 +76,10 @@ struct DevicePool {
         }
         void* p = nullptr;
         if (alloc(&p, bytes) != hipSuccess) {
-            // drop cached blocks and retry once
-            for (auto& kv : free_) release(kv.second);
+            for (auto& kv : free_) {
+                release(kv.second);
+                sizes_.erase(kv.second);
+            }
             free_.clear();
             if (alloc(&p, bytes) != hipSuccess) return nullptr;
         }
@@ -91,8 +95,13 @@ struct DevicePool {
         }
         free_.emplace(it->second, p);
     }
-    std::map<void*, size_t> sizes_;
 };
+
+int32_t next_pow2(int64_t v) {
+    int64_t n = 1;
+    while (n < v) n <<= 1;
+    return (int32_t)n;
+}
 
 }  // namespace
 
@@ -102,9 +111,9 @@ struct bpx_ctx {
     int num_cus = 0;
     int64_t lds_limit = 0;
     int64_t hbm_bytes = 0;
-    uint32_t* work_counter = nullptr;
-    DevicePool pool;
-    DevicePool pinned{true};
+    uint32_t* work_counter = nullptr;  // one per launch class
+    Pool pool;
+    Pool pinned{true};
 };
 
 struct bpx_db {
@@ -115,206 +124,532 @@ struct bpx_db {
     uint8_t* d_arena = nullptr;
     uint64_t* d_off = nullptr;
     uint32_t* d_len = nullptr;
-    uint32_t* d_order = nullptr;        // all subjects, longest first
+    uint32_t* d_order = nullptr;  // all subjects, longest first
     std::vector<uint32_t> h_len;
     std::vector<uint32_t> h_order;
 };
 
-struct bpx_query {
+// Queries staged on the device as scan groups.
+struct QuerySet {
     bpx_ctx* ctx = nullptr;
+    uint32_t nq = 0;
+    int32_t word = 3, window = 40;
+    uint32_t ngroups_pv = 0;
+    std::vector<bpx::GroupInfo> groups;
+    std::vector<bpx::QueryInfo> qinfo;
+    std::vector<uint32_t> max_cell;  // per group
+    uint64_t table_bytes = 0;
+    bpx::GroupInfo* d_groups = nullptr;
+    bpx::QueryInfo* d_qinfo = nullptr;
+    uint32_t* d_tables = nullptr;
+    uint8_t* d_qres = nullptr;
+    int8_t* d_matrix = nullptr;
+
+    void release() {
+        (void)hipFree(d_groups);
+        (void)hipFree(d_qinfo);
+        (void)hipFree(d_tables);
+        (void)hipFree(d_qres);
+        (void)hipFree(d_matrix);
+        d_groups = nullptr;
+        d_qinfo = nullptr;
+        d_tables = nullptr;
+        d_qres = nullptr;
+        d_matrix = nullptr;
+    }
+};
+
+struct bpx_query {
+    QuerySet set;
     int32_t qlen = 0;
     bpx_word_params params{};
-    bpx::HostTable table;
-    uint64_t table_bytes = 0;
-    uint32_t* d_table = nullptr;
-    uint8_t* d_query = nullptr;
-    int8_t* d_matrix = nullptr;
+    int32_t npresent = 0;
+    int64_t nentries = 0;
+};
+
+struct bpx_batch {
+    QuerySet set;
 };
 
 struct bpx_result {
     bpx_ctx* ctx = nullptr;
+    const QuerySet* set = nullptr;
+    const bpx_db* db = nullptr;
     uint32_t first = 0, count = 0;
     uint32_t flags = 0;
+    bpx_launch launch{};
     bool waited = false;
+    int reruns = 0;
     // device
+    uint32_t* d_cnt = nullptr;
     int32_t* d_stats = nullptr;
-    uint32_t* d_rec_loc = nullptr;
-    bpx_init_hsp* d_seg = nullptr;
-    uint32_t seg_cap = 0, seg_waves = 0;
+    uint32_t* d_run_loc = nullptr;
+    uint32_t* d_run_len = nullptr;
+    bpx::SegRecord* d_seg = nullptr;
+    uint32_t seg_cap = 0;
+    uint64_t seg_total = 0;
     uint32_t* d_tile = nullptr;
-    uint32_t* d_rec_off = nullptr;
     bpx_init_hsp* d_records = nullptr;
     uint32_t* d_subjects = nullptr;
-    unsigned long long* d_ustats = nullptr;  // 5 stats + total + overflow word
-    uint32_t* d_overflow = nullptr;
-    uint32_t* d_order = nullptr;             // owned only for sub-range scans
-    // host
-    unsigned long long* h_ustats = nullptr;  // pinned: 5 stats, total, overflow
-    bpx_init_hsp* h_records = nullptr;     // pinned
-    uint32_t* h_subjects = nullptr;        // pinned
-    bpx_subject_stats* h_sstats = nullptr; // pinned
-    bpx_ungapped_stats ustats{};
+    uint32_t* d_queries = nullptr;
+    unsigned long long* d_meta = nullptr;  // [0] total, [1] overflow word, [8..] 5 stats per query
+    uint32_t* d_order = nullptr;           // owned only for sub-range scans
+    bpx::WorkItem* d_items = nullptr;
+    // host (pinned)
+    unsigned long long* h_meta = nullptr;
+    bpx_init_hsp* h_records = nullptr;
+    uint32_t* h_subjects = nullptr;
+    uint32_t* h_queries = nullptr;
+    bpx_subject_stats* h_sstats = nullptr;
     uint64_t total = 0;
     // timing
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float kernel_ms = 0.f, device_ms = 0.f;
-    // relaunch state (segment overflow)
-    const bpx_query* q = nullptr;
-    const bpx_db* db = nullptr;
-    bpx_launch launch{};
-    int reruns = 0;
+    // launch plan
+    struct Class {
+        bool tab_lds = false, multi = false;
+        int waves = 0;
+        bpx::LdsPlan plan{};
+        uint32_t item_first = 0, nitems = 0;
+        uint32_t diag_cap = 0;
+    };
+    std::vector<Class> classes;
+    std::vector<bpx::WorkItem> items;
+    uint32_t qcap = 0;
 };
 
 namespace {
 
-void result_release(bpx_result* r) {
-    auto& pool = r->ctx->pool;
-    pool.put(r->d_stats);
-    pool.put(r->d_rec_loc);
-    pool.put(r->d_seg);
-    pool.put(r->d_tile);
-    pool.put(r->d_rec_off);
-    pool.put(r->d_records);
-    pool.put(r->d_subjects);
-    pool.put(r->d_ustats);
-    pool.put(r->d_order);
-    r->ctx->pinned.put(r->h_records);
-    r->ctx->pinned.put(r->h_subjects);
-    r->ctx->pinned.put(r->h_sstats);
-    r->h_records = nullptr;
-    r->h_subjects = nullptr;
-    r->h_sstats = nullptr;
-    r->d_stats = nullptr;
-    r->d_rec_loc = nullptr;
-    r->d_seg = nullptr;
-    r->d_tile = nullptr;
-    r->d_rec_off = nullptr;
-    r->d_records = nullptr;
-    r->d_subjects = nullptr;
-    r->d_ustats = nullptr;
-    r->d_overflow = nullptr;
-    r->d_order = nullptr;
-}
+// ---- query sets -------------------------------------------------------------
 
-bpx_launch resolve_launch(const bpx_launch* in, const bpx::HostTable& t, int32_t qlen = 0,
-                          int32_t diag_len = 0, int64_t lds_limit = 0) {
-    bpx_launch l{};
-    if (in) l = *in;
-    const bool auto_waves = l.waves_per_group <= 0;
-    if (auto_waves) l.waves_per_group = 8;
-    l.waves_per_group = std::min(16, std::max(1, l.waves_per_group));
-    if (l.groups_per_cu <= 0) l.groups_per_cu = 1;
-    if (l.subjects_per_grab <= 0) l.subjects_per_grab = 1;
-    uint32_t qc = l.queue_capacity > 0 ? (uint32_t)l.queue_capacity : 256u;
-    const uint32_t need = std::max<uint32_t>(128u, t.max_cell + 64u);
-    while (qc < need) qc <<= 1;
-    uint32_t p2 = 1;
-    while (p2 < qc) p2 <<= 1;
-    l.queue_capacity = (int32_t)p2;
-    // Long queries have large diagonal tables: with the default shape, use
-    // fewer wavefronts per workgroup until the per-wave LDS fits.
-    if (auto_waves && lds_limit > 0)
-        while (l.waves_per_group > 1 &&
-               bpx::plan_lds((uint32_t)t.packed_bytes(), qlen, diag_len, p2, l.waves_per_group,
-                             (uint32_t)lds_limit, true).total > (uint64_t)lds_limit)
-            l.waves_per_group /= 2;
-    return l;
-}
+struct QuerySpec {
+    const uint8_t* seq;
+    int32_t len;
+    int32_t xdrop, cutoff, diag_len;
+};
 
-int32_t diag_len_of(const bpx_word_params& p) { return p.diag_mask + 1; }
-
-// Enqueues the word finder and the compaction for result r (buffers already
-// allocated).
-int enqueue_scan(bpx_result* r, const bpx_query* q, const bpx_db* db, const bpx_launch& L,
-                 const bpx::LdsPlan& plan, const uint32_t* order) {
-    bpx_ctx* ctx = r->ctx;
-    HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_ustats), 0u, 16, ctx->stream));
-    HIP_TRY(bpx::launch_fill_u32(ctx->work_counter, 0u, 1, ctx->stream));
-
-    bpx::ScanParams sp{};
-    sp.residues = db->d_arena;
-    sp.soff = db->d_off;
-    sp.slen = db->d_len;
-    sp.order = order;
-    sp.norder = r->count;
-    sp.first = r->first;
-    sp.table = q->d_table;
-    sp.table_words = (uint32_t)(q->table_bytes / 4);
-    sp.ngroups = q->table.ngroups;
-    sp.npresent = q->table.npresent;
-    sp.query = q->d_query;
-    sp.qlen = q->qlen;
-    sp.matrix = q->d_matrix;
-    sp.word = q->params.word_length;
-    sp.window = q->params.window;
-    sp.xdrop = q->params.x_dropoff;
-    sp.cutoff = q->params.cutoff_score;
-    sp.diag_mask = q->params.diag_mask;
-    sp.work_counter = ctx->work_counter;
-    sp.grab = (uint32_t)L.subjects_per_grab;
-    sp.stats = r->d_stats;
-    sp.rec_loc = r->d_rec_loc;
-    sp.seg = r->d_seg;
-    sp.seg_cap = r->seg_cap;
-    sp.overflow = r->d_overflow;
-    sp.lds_table = plan.table;
-    sp.lds_matrix = plan.matrix;
-    sp.lds_query = plan.query;
-    sp.lds_wave = plan.wave;
-    sp.wave_stride = plan.wave_stride;
-    sp.qcap = (uint32_t)L.queue_capacity;
-    sp.lds_total = plan.total;
-
-    const int groups = ctx->num_cus * L.groups_per_cu;
-    HIP_TRY(hipEventRecord(r->ev[0], ctx->stream));
-    if (r->count > 0) HIP_TRY(bpx::launch_scan(sp, plan.table_in_lds, L.waves_per_group, groups, ctx->stream));
-    HIP_TRY(hipEventRecord(r->ev[1], ctx->stream));
-
-    bpx::CompactParams cp{};
-    cp.stats = r->d_stats;
-    cp.rec_loc = r->d_rec_loc;
-    cp.seg = r->d_seg;
-    cp.count = r->count;
-    cp.first = r->first;
-    cp.tile_sum = r->d_tile;
-    cp.rec_off = r->d_rec_off;
-    cp.records = r->d_records;
-    cp.subjects = r->d_subjects;
-    cp.ustats = r->d_ustats;
-    cp.overflow = r->d_overflow;
-    HIP_TRY(bpx::launch_compact(cp, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(r->h_ustats, r->d_ustats, 7 * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    if ((r->flags & BPX_OUT_SUBJECT_STATS) && r->count) {
-        if (!r->h_sstats)
-            r->h_sstats = (bpx_subject_stats*)ctx->pinned.get((size_t)r->count * sizeof(bpx_subject_stats));
-        if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
-        HIP_TRY(hipMemcpyAsync(r->h_sstats, r->d_stats, (size_t)r->count * sizeof(bpx_subject_stats),
-                               hipMemcpyDeviceToHost, ctx->stream));
+int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, int32_t word, int32_t threshold,
+              int32_t window, int32_t group_residues, int32_t group_queries, QuerySet* out,
+              std::vector<bpx::HostTable>* keep_tables = nullptr) {
+    QuerySet& set = *out;
+    set.ctx = ctx;
+    set.nq = (uint32_t)qs.size();
+    set.word = word;
+    set.window = window;
+    set.ngroups_pv = (1u << (5 * word)) / 32;
+    // Pack short queries (ascending length) into groups; long ones alone.
+    std::vector<uint32_t> order(qs.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return qs[a].len < qs[b].len; });
+    std::vector<std::vector<uint32_t>> members;
+    for (size_t k = 0; k < order.size();) {
+        std::vector<uint32_t> m{order[k]};
+        int64_t tot = qs[order[k]].len;
+        ++k;
+        if (tot < group_residues)
+            while (k < order.size() && (int32_t)m.size() < group_queries && tot + qs[order[k]].len <= group_residues) {
+                tot += qs[order[k]].len;
+                m.push_back(order[k]);
+                ++k;
+            }
+        members.push_back(std::move(m));
+    }
+    std::vector<uint32_t> tables;
+    std::vector<uint8_t> qres;
+    for (const auto& m : members) {
+        std::vector<bpx::QuerySpan> spans;
+        bpx::GroupInfo gi{};
+        gi.qinfo_first = (uint32_t)set.qinfo.size();
+        gi.nq = (uint32_t)m.size();
+        gi.qres_off = (uint32_t)qres.size();
+        uint32_t qstart = 0, dbase = 0;
+        for (uint32_t id : m) {
+            const QuerySpec& q = qs[id];
+            spans.push_back({q.seq, q.len});
+            bpx::QueryInfo qi{};
+            qi.qstart = qstart;
+            qi.qlen = (uint32_t)q.len;
+            qi.dbase = dbase;
+            qi.dmask = (uint32_t)q.diag_len - 1;
+            qi.xdrop = q.xdrop;
+            qi.cutoff = q.cutoff;
+            qi.qid = id;
+            set.qinfo.push_back(qi);
+            qres.insert(qres.end(), q.seq, q.seq + q.len);
+            qstart += (uint32_t)q.len;
+            dbase += (uint32_t)q.diag_len;
+        }
+        gi.qres_len = qstart;
+        gi.diag_slots = dbase;
+        bpx::HostTable t;
+        if (!bpx::build_table_multi(spans.data(), (int32_t)spans.size(), M, threshold, word, &t))
+            return fail(BPX_E_ARG, "lookup table build failed (residue codes must be < 28, group < 65536 residues)");
+        const std::vector<uint32_t> blob = t.pack();
+        gi.table_word = (uint32_t)tables.size();
+        gi.table_words = (uint32_t)blob.size();
+        gi.npresent = t.npresent;
+        tables.insert(tables.end(), blob.begin(), blob.end());
+        set.max_cell.push_back(t.max_cell);
+        set.groups.push_back(gi);
+        if (keep_tables) keep_tables->push_back(std::move(t));
+    }
+    set.table_bytes = (uint64_t)tables.size() * 4;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t ng = std::max<size_t>(set.groups.size(), 1), nqi = std::max<size_t>(set.qinfo.size(), 1);
+    if (hipMalloc(&set.d_groups, ng * sizeof(bpx::GroupInfo)) != hipSuccess ||
+        hipMalloc(&set.d_qinfo, nqi * sizeof(bpx::QueryInfo)) != hipSuccess ||
+        hipMalloc(&set.d_tables, std::max<size_t>(tables.size(), 1) * 4) != hipSuccess ||
+        hipMalloc(&set.d_qres, std::max<size_t>(qres.size(), 1)) != hipSuccess ||
+        hipMalloc(&set.d_matrix, 28 * 28) != hipSuccess) {
+        set.release();
+        return fail(BPX_E_NOMEM, "device allocation for query tables failed");
+    }
+    hipError_t e = hipSuccess;
+    if (!set.groups.empty())
+        e = hipMemcpy(set.d_groups, set.groups.data(), set.groups.size() * sizeof(bpx::GroupInfo),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && !set.qinfo.empty())
+        e = hipMemcpy(set.d_qinfo, set.qinfo.data(), set.qinfo.size() * sizeof(bpx::QueryInfo), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tables.empty())
+        e = hipMemcpy(set.d_tables, tables.data(), tables.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !qres.empty()) e = hipMemcpy(set.d_qres, qres.data(), qres.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(set.d_matrix, M, 28 * 28, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set.release();
+        return fail(BPX_E_HIP, "query upload failed: %s", hipGetErrorString(e));
     }
     return BPX_OK;
 }
 
-int alloc_result_buffers(bpx_result* r, uint32_t seg_cap, uint32_t seg_waves) {
+// ---- launch planning ----------------------------------------------------------
+
+bpx_launch resolve_launch(const bpx_launch* in) {
+    bpx_launch l{};
+    if (in) l = *in;
+    if (l.groups_per_cu <= 0) l.groups_per_cu = 1;
+    if (l.subject_buffer <= 0) l.subject_buffer = 2048;
+    uint32_t sb = 1024;
+    while ((int32_t)sb < l.subject_buffer) sb <<= 1;
+    l.subject_buffer = (int32_t)sb;
+    return l;
+}
+
+// Splits the set's groups into launch classes and lays out LDS for each.
+int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_launch* user) {
+    bpx_ctx* ctx = r->ctx;
+    const bpx_launch L = resolve_launch(user);
+    const bool auto_waves = !user || user->waves_per_group <= 0;
+    const int want_waves = auto_waves ? 8 : std::min(16, std::max(1, user->waves_per_group));
+    uint32_t maxcell = 0;
+    for (uint32_t c : set.max_cell) maxcell = std::max(maxcell, c);
+    uint32_t qc = L.queue_capacity > 0 ? (uint32_t)L.queue_capacity : 256u;
+    qc = std::max<uint32_t>(qc, std::max<uint32_t>(128u, maxcell + 64u));
+    uint32_t qcap = 1;
+    while (qcap < qc) qcap <<= 1;
+    r->qcap = qcap;
+    const uint32_t sbuf = (uint32_t)L.subject_buffer;
+    const uint32_t limit = (uint32_t)ctx->lds_limit;
+
+    struct Acc {
+        std::vector<uint32_t> groups;
+        uint32_t table = 0, qres = 0, nq = 1, diag = 0;
+    };
+    auto plan_of = [&](const Acc& a, bool tab_lds, int waves) {
+        return bpx::plan_lds(a.table, a.qres, a.nq, a.diag, qcap, sbuf, waves, tab_lds);
+    };
+    Acc acc[kMaxClasses];  // index: tab_lds * 2 + multi
+    // groups by table size: the LDS class takes the smaller tables first
+    std::vector<uint32_t> gorder(set.groups.size());
+    std::iota(gorder.begin(), gorder.end(), 0u);
+    std::stable_sort(gorder.begin(), gorder.end(),
+                     [&](uint32_t a, uint32_t b) { return set.groups[a].table_words < set.groups[b].table_words; });
+    for (uint32_t gi : gorder) {
+        const bpx::GroupInfo& g = set.groups[gi];
+        const int multi = g.nq > 1 ? 1 : 0;
+        Acc t = acc[2 + multi];
+        t.groups.push_back(gi);
+        t.table = std::max(t.table, g.table_words * 4);
+        t.qres = std::max(t.qres, g.qres_len);
+        t.nq = std::max(t.nq, g.nq);
+        t.diag = std::max(t.diag, g.diag_slots);
+        // keep the group in the LDS class if at least one wave still fits
+        if (plan_of(t, true, 1).total <= limit) {
+            acc[2 + multi] = t;
+        } else {
+            Acc& h = acc[multi];
+            h.groups.push_back(gi);
+            h.qres = std::max(h.qres, g.qres_len);
+            h.nq = std::max(h.nq, g.nq);
+            h.diag = std::max(h.diag, g.diag_slots);
+        }
+    }
+    r->classes.clear();
+    r->items.clear();
+    const uint32_t R =
+        L.subjects_per_item > 0
+            ? (uint32_t)L.subjects_per_item
+            : std::max<uint32_t>(64u, std::min<uint32_t>(4096u, count / (uint32_t)std::max(1, ctx->num_cus * 4) + 1));
+    for (int cls = kMaxClasses - 1; cls >= 0; --cls) {
+        const Acc& a = acc[cls];
+        if (a.groups.empty()) continue;
+        bpx_result::Class c;
+        c.tab_lds = cls >= 2;
+        c.multi = (cls & 1) != 0;
+        int waves = want_waves;
+        bpx::LdsPlan pl = plan_of(a, c.tab_lds, waves);
+        while (auto_waves && waves > 1 && pl.total > limit) {
+            waves /= 2;
+            pl = plan_of(a, c.tab_lds, waves);
+        }
+        if (pl.total > limit)
+            return fail(BPX_E_BUDGET,
+                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
+                        "use fewer waves per group or a smaller subject buffer",
+                        pl.total, limit, a.diag, waves, sbuf);
+        c.waves = waves;
+        c.plan = pl;
+        c.diag_cap = std::max<uint32_t>(a.diag, 1);
+        c.item_first = (uint32_t)r->items.size();
+        for (uint32_t gi : a.groups)
+            for (uint32_t f = 0; f < count; f += R) r->items.push_back({gi, f, std::min(R, count - f), 0});
+        c.nitems = (uint32_t)r->items.size() - c.item_first;
+        r->classes.push_back(c);
+    }
+    r->launch = L;
+    return BPX_OK;
+}
+
+// ---- results --------------------------------------------------------------------
+
+void result_release(bpx_result* r, bool keep_order) {
     auto& pool = r->ctx->pool;
-    const uint32_t cnt = std::max<uint32_t>(r->count, 1);
-    const uint32_t ntiles = (cnt + bpx::kTile - 1) / bpx::kTile;
-    const uint64_t seg_total = (uint64_t)seg_cap * seg_waves;
+    auto& pinned = r->ctx->pinned;
+    pool.put(r->d_cnt);
+    pool.put(r->d_stats);
+    pool.put(r->d_run_loc);
+    pool.put(r->d_run_len);
+    pool.put(r->d_seg);
+    pool.put(r->d_tile);
+    pool.put(r->d_records);
+    pool.put(r->d_subjects);
+    pool.put(r->d_queries);
+    pool.put(r->d_meta);
+    pool.put(r->d_items);
+    if (!keep_order) {
+        pool.put(r->d_order);
+        r->d_order = nullptr;
+    }
+    pinned.put(r->h_records);
+    pinned.put(r->h_subjects);
+    pinned.put(r->h_queries);
+    pinned.put(r->h_sstats);
+    pinned.put(r->h_meta);
+    r->d_cnt = nullptr;
+    r->d_stats = nullptr;
+    r->d_run_loc = nullptr;
+    r->d_run_len = nullptr;
+    r->d_seg = nullptr;
+    r->d_tile = nullptr;
+    r->d_records = nullptr;
+    r->d_subjects = nullptr;
+    r->d_queries = nullptr;
+    r->d_meta = nullptr;
+    r->d_items = nullptr;
+    r->h_records = nullptr;
+    r->h_subjects = nullptr;
+    r->h_queries = nullptr;
+    r->h_sstats = nullptr;
+    r->h_meta = nullptr;
+}
+
+uint64_t total_waves(const bpx_result* r) {
+    uint64_t waves = 0;
+    for (const auto& c : r->classes) waves += (uint64_t)r->ctx->num_cus * r->launch.groups_per_cu * c.waves;
+    return waves;
+}
+
+int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
+    auto& pool = r->ctx->pool;
+    const QuerySet& set = *r->set;
+    const uint64_t pairs = (uint64_t)set.nq * r->count;
+    const uint64_t runs = (uint64_t)set.groups.size() * r->count;
     r->seg_cap = seg_cap;
-    r->seg_waves = seg_waves;
-    r->d_stats = (int32_t*)pool.get((size_t)cnt * 3 * sizeof(int32_t));
-    r->d_rec_loc = (uint32_t*)pool.get((size_t)cnt * sizeof(uint32_t));
-    r->d_seg = (bpx_init_hsp*)pool.get(seg_total * sizeof(bpx_init_hsp));
-    r->d_tile = (uint32_t*)pool.get((size_t)ntiles * sizeof(uint32_t));
-    r->d_rec_off = (uint32_t*)pool.get(((size_t)cnt + 1) * sizeof(uint32_t));
-    r->d_records = (bpx_init_hsp*)pool.get(seg_total * sizeof(bpx_init_hsp));
-    r->d_subjects = (uint32_t*)pool.get(seg_total * sizeof(uint32_t));
-    r->d_ustats = (unsigned long long*)pool.get(8 * sizeof(unsigned long long));
-    if (!r->d_stats || !r->d_rec_loc || !r->d_seg || !r->d_tile || !r->d_rec_off || !r->d_records ||
-        !r->d_subjects || !r->d_ustats)
-        return fail(BPX_E_NOMEM, "device allocation for scan results failed (%llu segment records)",
-                    (unsigned long long)seg_total);
-    r->d_overflow = reinterpret_cast<uint32_t*>(r->d_ustats + 6);
+    r->seg_total = total_waves(r) * seg_cap;
+    if (r->seg_total > 0xFFFFFFFFull) return fail(BPX_E_BUDGET, "record segments exceed 2^32 records");
+    const uint64_t ntiles = (pairs + bpx::kTile - 1) / bpx::kTile;
+    r->d_cnt = (uint32_t*)pool.get(pairs * 4);
+    if (r->flags & BPX_OUT_SUBJECT_STATS) r->d_stats = (int32_t*)pool.get(pairs * 12);
+    r->d_run_loc = (uint32_t*)pool.get(runs * 4);
+    r->d_run_len = (uint32_t*)pool.get(runs * 4);
+    r->d_seg = (bpx::SegRecord*)pool.get(r->seg_total * sizeof(bpx::SegRecord));
+    r->d_tile = (uint32_t*)pool.get(std::max<uint64_t>(ntiles, 1) * 4);
+    r->d_records = (bpx_init_hsp*)pool.get(r->seg_total * sizeof(bpx_init_hsp));
+    r->d_subjects = (uint32_t*)pool.get(r->seg_total * 4);
+    r->d_queries = (uint32_t*)pool.get(r->seg_total * 4);
+    r->d_meta = (unsigned long long*)pool.get((kMetaWords + 5ull * set.nq) * 8);
+    r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
+    r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
+    if (!r->d_cnt || ((r->flags & BPX_OUT_SUBJECT_STATS) && !r->d_stats) || !r->d_run_loc || !r->d_run_len ||
+        !r->d_seg || !r->d_tile || !r->d_records || !r->d_subjects || !r->d_queries || !r->d_meta || !r->d_items ||
+        !r->h_meta)
+        return fail(BPX_E_NOMEM, "allocation for scan results failed (%llu query-subject pairs, %llu segment records)",
+                    (unsigned long long)pairs, (unsigned long long)r->seg_total);
+    return BPX_OK;
+}
+
+int enqueue(bpx_result* r, const uint32_t* order) {
+    bpx_ctx* ctx = r->ctx;
+    const QuerySet& set = *r->set;
+    const bpx_db* db = r->db;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_meta), 0u, 2 * (kMetaWords + 5ull * set.nq), st));
+    HIP_TRY(bpx::launch_fill_u32(ctx->work_counter, 0u, kMaxClasses, st));
+    if (!r->items.empty())
+        HIP_TRY(hipMemcpyAsync(r->d_items, r->items.data(), r->items.size() * sizeof(bpx::WorkItem),
+                               hipMemcpyHostToDevice, st));
+    unsigned long long* ustats = r->d_meta + kMetaWords;
+    uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
+    HIP_TRY(hipEventRecord(r->ev[0], st));
+    uint64_t seg_at = 0;
+    for (size_t ci = 0; ci < r->classes.size(); ++ci) {
+        const auto& c = r->classes[ci];
+        const int groups = ctx->num_cus * r->launch.groups_per_cu;
+        bpx::ScanParams sp{};
+        sp.residues = db->d_arena;
+        sp.soff = db->d_off;
+        sp.slen = db->d_len;
+        sp.order = order;
+        sp.first = r->first;
+        sp.count = r->count;
+        sp.groups = set.d_groups;
+        sp.qinfo = set.d_qinfo;
+        sp.tables = set.d_tables;
+        sp.qres = set.d_qres;
+        sp.matrix = set.d_matrix;
+        sp.ngroups_pv = set.ngroups_pv;
+        sp.word = set.word;
+        sp.window = set.window;
+        sp.items = r->d_items + c.item_first;
+        sp.nitems = c.nitems;
+        sp.work_counter = ctx->work_counter + ci;
+        sp.cnt = r->d_cnt;
+        sp.stats = r->d_stats;
+        sp.run_loc = r->d_run_loc;
+        sp.run_len = r->d_run_len;
+        sp.ustats = ustats;
+        // each launch class owns its own range of wave segments; run
+        // locations written by the kernel are relative to sp.seg, so they
+        // carry the class offset below
+        sp.seg = r->d_seg;
+        sp.seg_cap = r->seg_cap;
+        sp.seg_wave_base = (uint32_t)(seg_at / r->seg_cap);
+        sp.overflow = overflow;
+        sp.lds_table = c.plan.table;
+        sp.lds_qres = c.plan.qres;
+        sp.lds_qmap = c.plan.qmap;
+        sp.lds_qinfo = c.plan.qinfo;
+        sp.lds_matrix = c.plan.matrix;
+        sp.lds_wave = c.plan.wave;
+        sp.wave_stride = c.plan.wave_stride;
+        sp.lds_total = c.plan.total;
+        sp.table_cap_words = c.plan.table_cap_words;
+        sp.qres_cap = c.plan.qres_cap;
+        sp.max_nq = c.plan.max_nq;
+        sp.diag_cap = c.diag_cap;
+        sp.qcap = r->qcap;
+        sp.sbuf = (uint32_t)r->launch.subject_buffer;
+        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.waves, groups, st));
+        seg_at += (uint64_t)groups * c.waves * r->seg_cap;
+    }
+    HIP_TRY(hipEventRecord(r->ev[1], st));
+    bpx::CompactParams cp{};
+    cp.cnt = r->d_cnt;
+    cp.n = (uint64_t)set.nq * r->count;
+    cp.tile_sum = r->d_tile;
+    cp.run_loc = r->d_run_loc;
+    cp.run_len = r->d_run_len;
+    cp.groups = set.d_groups;
+    cp.qinfo = set.d_qinfo;
+    cp.ngroups = (uint32_t)set.groups.size();
+    cp.count = r->count;
+    cp.first = r->first;
+    cp.seg = r->d_seg;
+    cp.records = r->d_records;
+    cp.subjects = r->d_subjects;
+    cp.queries = r->d_queries;
+    cp.total = r->d_meta;
+    cp.overflow = overflow;
+    HIP_TRY(bpx::launch_compact(cp, st));
+    HIP_TRY(hipMemcpyAsync(r->h_meta, r->d_meta, (kMetaWords + 5ull * set.nq) * 8, hipMemcpyDeviceToHost, st));
+    if ((r->flags & BPX_OUT_SUBJECT_STATS) && r->count) {
+        const uint64_t pairs = (uint64_t)set.nq * r->count;
+        if (!r->h_sstats) r->h_sstats = (bpx_subject_stats*)ctx->pinned.get(pairs * sizeof(bpx_subject_stats));
+        if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
+        HIP_TRY(hipMemcpyAsync(r->h_sstats, r->d_stats, pairs * sizeof(bpx_subject_stats), hipMemcpyDeviceToHost, st));
+    }
+    return BPX_OK;
+}
+
+int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t first, uint32_t count,
+               const bpx_launch* launch, uint32_t out_flags, bpx_result** out) {
+    if (!ctx || !db || !out) return fail(BPX_E_ARG, "NULL handle");
+    *out = nullptr;
+    if (set.ctx != ctx || db->ctx != ctx) return fail(BPX_E_ARG, "handles belong to another context");
+    if ((uint64_t)first + count > db->n) return fail(BPX_E_ARG, "subject range outside the database");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* r = new (std::nothrow) bpx_result;
+    if (!r) return fail(BPX_E_NOMEM, "host allocation failed");
+    r->ctx = ctx;
+    r->set = &set;
+    r->db = db;
+    r->first = first;
+    r->count = count;
+    r->flags = out_flags;
+    int rc = plan_scan(r, set, count, launch);
+    if (rc) {
+        delete r;
+        return rc;
+    }
+    for (auto& ev : r->ev)
+        if (hipEventCreate(&ev) != hipSuccess) {
+            bpx_result_destroy(r);
+            return fail(BPX_E_HIP, "event creation failed");
+        }
+    const uint64_t waves = std::max<uint64_t>(total_waves(r), 1);
+    const uint64_t pairs = (uint64_t)set.nq * count;
+    const uint32_t seg_cap = (uint32_t)std::max<uint64_t>(64, (pairs / 8 + 65536) / waves + 1);
+    rc = alloc_buffers(r, seg_cap);
+    if (rc) {
+        bpx_result_destroy(r);
+        return rc;
+    }
+    const uint32_t* order = db->d_order;
+    if (!(first == 0 && count == db->n)) {
+        std::vector<uint32_t> sub;
+        sub.reserve(count);
+        for (uint32_t id : db->h_order)
+            if (id >= first && id - first < count) sub.push_back(id);
+        r->d_order = (uint32_t*)ctx->pool.get(std::max<size_t>(count, 1) * 4);
+        if (!r->d_order) {
+            bpx_result_destroy(r);
+            return fail(BPX_E_NOMEM, "work list allocation failed");
+        }
+        if (count) {
+            hipError_t e = hipMemcpyAsync(r->d_order, sub.data(), (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                bpx_result_destroy(r);
+                return fail(BPX_E_HIP, "work list upload failed: %s", hipGetErrorString(e));
+            }
+        }
+        order = r->d_order;
+    }
+    rc = enqueue(r, order);
+    if (rc) {
+        bpx_result_destroy(r);
+        return rc;
+    }
+    *out = r;
     return BPX_OK;
 }
 
@@ -387,7 +722,7 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
     std::vector<uint64_t> off(num_subjects);
     uint64_t at = 0, nres = 0;
     for (uint32_t i = 0; i < num_subjects; ++i) {
-        if (lengths[i] > 0x7FFFFFFFu) return fail(BPX_E_ARG, "subject %u too long", i);
+        if (lengths[i] >= (1u << 28)) return fail(BPX_E_ARG, "subject %u longer than 2^28 residues", i);
         off[i] = at;
         at += ((uint64_t)lengths[i] + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
         nres += lengths[i];
@@ -410,8 +745,8 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
     db->h_len.assign(lengths, lengths + num_subjects);
     db->h_order.resize(num_subjects);
     std::iota(db->h_order.begin(), db->h_order.end(), 0u);
-    // longest first (ties by index): the scan's dynamic queue then behaves
-    // like longest-processing-time-first list scheduling
+    // longest first (ties by index): the dynamic work queue then behaves like
+    // longest-processing-time-first list scheduling
     std::stable_sort(db->h_order.begin(), db->h_order.end(),
                      [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
     const size_t n1 = std::max<uint32_t>(num_subjects, 1);
@@ -463,39 +798,23 @@ int bpx_query_prepare(bpx_ctx* ctx, const uint8_t* query, int32_t qlen, const in
     bpx_word_params p = *params;
     if (p.word_length != 2 && p.word_length != 3) return fail(BPX_E_ARG, "word_length must be 2 or 3");
     if (p.window <= 0 || p.x_dropoff <= 0) return fail(BPX_E_ARG, "window and x_dropoff must be positive");
-    if (p.diag_mask == 0) {
-        int32_t n = 1;
-        while (n < qlen + p.window) n <<= 1;
-        p.diag_mask = n - 1;
-    }
-    if (p.diag_mask < 0 || ((p.diag_mask + 1) & p.diag_mask) != 0 || p.diag_mask + 1 > (1 << 16))
-        return fail(BPX_E_ARG, "diag_mask must be 2^k - 1 with 2^k <= 65536");
+    if (p.diag_mask == 0) p.diag_mask = next_pow2((int64_t)qlen + p.window) - 1;
+    if (p.diag_mask < 0 || ((p.diag_mask + 1) & p.diag_mask) != 0 || p.diag_mask + 1 > (1 << 17))
+        return fail(BPX_E_ARG, "diag_mask must be 2^k - 1 with 2^k <= 131072");
     const int8_t* M = matrix ? matrix : bpx::blosum62();
-    HIP_TRY(hipSetDevice(ctx->device));
     auto* q = new (std::nothrow) bpx_query;
     if (!q) return fail(BPX_E_NOMEM, "host allocation failed");
-    q->ctx = ctx;
     q->qlen = qlen;
     q->params = p;
-    if (!bpx::build_table(query, qlen, M, p.threshold, p.word_length, &q->table)) {
+    std::vector<QuerySpec> qs{{query, qlen, p.x_dropoff, p.cutoff_score, p.diag_mask + 1}};
+    std::vector<bpx::HostTable> tabs;
+    int rc = build_set(ctx, qs, M, p.word_length, p.threshold, p.window, 0, 1, &q->set, &tabs);
+    if (rc) {
         delete q;
-        return fail(BPX_E_ARG, "lookup table build failed (residue codes must be < 28)");
+        return rc;
     }
-    const std::vector<uint32_t> blob = q->table.pack();
-    q->table_bytes = (uint64_t)blob.size() * 4;
-    const size_t qbytes = std::max<size_t>((size_t)qlen, 1);
-    if (hipMalloc(&q->d_table, q->table_bytes) != hipSuccess || hipMalloc(&q->d_query, qbytes) != hipSuccess ||
-        hipMalloc(&q->d_matrix, 28 * 28) != hipSuccess) {
-        bpx_query_destroy(q);
-        return fail(BPX_E_NOMEM, "device allocation for the query table failed");
-    }
-    hipError_t e = hipMemcpy(q->d_table, blob.data(), q->table_bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess && qlen) e = hipMemcpy(q->d_query, query, (size_t)qlen, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(q->d_matrix, M, 28 * 28, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        bpx_query_destroy(q);
-        return fail(BPX_E_HIP, "query upload failed: %s", hipGetErrorString(e));
-    }
+    q->npresent = (int32_t)tabs[0].npresent;
+    q->nentries = (int64_t)tabs[0].entries.size();
     *out = q;
     return BPX_OK;
 }
@@ -503,25 +822,74 @@ int bpx_query_prepare(bpx_ctx* ctx, const uint8_t* query, int32_t qlen, const in
 int bpx_query_info(const bpx_query* q, int64_t* table_bytes, int32_t* lds_resident, int32_t* num_present_words,
                    int64_t* num_entries) {
     if (!q) return fail(BPX_E_ARG, "query is NULL");
-    if (table_bytes) *table_bytes = (int64_t)q->table_bytes;
+    if (table_bytes) *table_bytes = (int64_t)q->set.table_bytes;
     if (lds_resident) {
-        const bpx_launch L = resolve_launch(nullptr, q->table, q->qlen, diag_len_of(q->params), q->ctx->lds_limit);
-        const bpx::LdsPlan pl = bpx::plan_lds((uint32_t)q->table_bytes, q->qlen, diag_len_of(q->params),
-                                              (uint32_t)L.queue_capacity, L.waves_per_group,
-                                              (uint32_t)q->ctx->lds_limit, true);
-        *lds_resident = pl.table_in_lds ? 1 : 0;
+        bpx_result tmp;
+        tmp.ctx = q->set.ctx;
+        *lds_resident = 0;
+        if (plan_scan(&tmp, q->set, 1, nullptr) == BPX_OK && !tmp.classes.empty())
+            *lds_resident = tmp.classes[0].tab_lds ? 1 : 0;
     }
-    if (num_present_words) *num_present_words = (int32_t)q->table.npresent;
-    if (num_entries) *num_entries = (int64_t)q->table.entries.size();
+    if (num_present_words) *num_present_words = q->npresent;
+    if (num_entries) *num_entries = q->nentries;
     return BPX_OK;
 }
 
 int bpx_query_destroy(bpx_query* q) {
     if (!q) return BPX_OK;
-    (void)hipFree(q->d_table);
-    (void)hipFree(q->d_query);
-    (void)hipFree(q->d_matrix);
+    q->set.release();
     delete q;
+    return BPX_OK;
+}
+
+int bpx_batch_prepare(bpx_ctx* ctx, uint32_t num_queries, const uint8_t* const* queries, const int32_t* qlens,
+                      const int8_t* matrix, const bpx_word_params* params, const int32_t* x_dropoff,
+                      const int32_t* cutoff, const bpx_batch_policy* policy, bpx_batch** out) {
+    if (!ctx || !out || !params) return fail(BPX_E_ARG, "ctx/params/out is NULL");
+    *out = nullptr;
+    if (num_queries && (!queries || !qlens)) return fail(BPX_E_ARG, "NULL query arrays");
+    const bpx_word_params& p = *params;
+    if (p.word_length != 2 && p.word_length != 3) return fail(BPX_E_ARG, "word_length must be 2 or 3");
+    if (p.window <= 0) return fail(BPX_E_ARG, "window must be positive");
+    if (p.diag_mask != 0) return fail(BPX_E_ARG, "a batch derives diag_mask per query; pass 0");
+    std::vector<QuerySpec> qs(num_queries);
+    for (uint32_t i = 0; i < num_queries; ++i) {
+        if (qlens[i] < 0 || qlens[i] > 65535 || (qlens[i] > 0 && !queries[i]))
+            return fail(BPX_E_ARG, "query %u: length %d outside [0, 65535]", i, qlens[i]);
+        qs[i].seq = queries[i];
+        qs[i].len = qlens[i];
+        qs[i].xdrop = x_dropoff ? x_dropoff[i] : p.x_dropoff;
+        qs[i].cutoff = cutoff ? cutoff[i] : p.cutoff_score;
+        if (qs[i].xdrop <= 0) return fail(BPX_E_ARG, "query %u: x_dropoff must be positive", i);
+        qs[i].diag_len = next_pow2((int64_t)qlens[i] + p.window);
+    }
+    int32_t gres = policy && policy->group_residues > 0 ? policy->group_residues : 600;
+    const int32_t gq = policy && policy->group_queries > 0 ? std::min(policy->group_queries, 255) : 32;
+    gres = std::min(gres, 65535);
+    auto* b = new (std::nothrow) bpx_batch;
+    if (!b) return fail(BPX_E_NOMEM, "host allocation failed");
+    int rc = build_set(ctx, qs, matrix ? matrix : bpx::blosum62(), p.word_length, p.threshold, p.window, gres, gq,
+                       &b->set);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return BPX_OK;
+}
+
+int bpx_batch_info(const bpx_batch* b, uint32_t* num_queries, uint32_t* num_groups, int64_t* table_bytes) {
+    if (!b) return fail(BPX_E_ARG, "batch is NULL");
+    if (num_queries) *num_queries = b->set.nq;
+    if (num_groups) *num_groups = (uint32_t)b->set.groups.size();
+    if (table_bytes) *table_bytes = (int64_t)b->set.table_bytes;
+    return BPX_OK;
+}
+
+int bpx_batch_destroy(bpx_batch* b) {
+    if (!b) return BPX_OK;
+    b->set.release();
+    delete b;
     return BPX_OK;
 }
 
@@ -529,18 +897,18 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
                      const bpx_launch* launch, size_t* device_bytes, size_t* lds_bytes) {
     if (!ctx || !db || !q) return fail(BPX_E_ARG, "NULL handle");
     if ((uint64_t)first + count > db->n) return fail(BPX_E_ARG, "subject range outside the database");
-    const bpx_launch L = resolve_launch(launch, q->table, q->qlen, diag_len_of(q->params), ctx->lds_limit);
-    const bpx::LdsPlan pl = bpx::plan_lds((uint32_t)q->table_bytes, q->qlen, diag_len_of(q->params),
-                                          (uint32_t)L.queue_capacity, L.waves_per_group,
-                                          (uint32_t)ctx->lds_limit, true);
-    const uint64_t waves = (uint64_t)ctx->num_cus * L.groups_per_cu * L.waves_per_group;
-    const uint64_t seg_cap = std::max<uint64_t>(64, ((uint64_t)count / 2 + 65536) / waves + 1);
-    const uint64_t need = db->arena_bytes + (uint64_t)db->n * 16 + q->table_bytes + (uint64_t)count * 24 +
-                          seg_cap * waves * (24 * 2 + 4);
+    bpx_result tmp;
+    tmp.ctx = ctx;
+    int rc = plan_scan(&tmp, q->set, count, launch);
+    uint32_t lds = 0;
+    for (const auto& c : tmp.classes) lds = std::max(lds, c.plan.total);
+    const uint64_t waves = std::max<uint64_t>(total_waves(&tmp), 1);
+    const uint64_t seg = std::max<uint64_t>(64, ((uint64_t)count / 8 + 65536) / waves + 1) * waves;
+    const uint64_t need = db->arena_bytes + (uint64_t)db->n * 16 + q->set.table_bytes + (uint64_t)count * 24 +
+                          seg * (sizeof(bpx::SegRecord) + sizeof(bpx_init_hsp) + 8);
     if (device_bytes) *device_bytes = (size_t)need;
-    if (lds_bytes) *lds_bytes = pl.total;
-    if (pl.total > (uint64_t)ctx->lds_limit)
-        return fail(BPX_E_BUDGET, "LDS plan %u B exceeds %lld B", pl.total, (long long)ctx->lds_limit);
+    if (lds_bytes) *lds_bytes = lds;
+    if (rc) return rc;
     if (ctx->hbm_bytes && need > (uint64_t)ctx->hbm_bytes)
         return fail(BPX_E_BUDGET, "needs %llu device bytes", (unsigned long long)need);
     return BPX_OK;
@@ -548,73 +916,14 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
 
 int bpx_scan(bpx_ctx* ctx, const bpx_query* q, const bpx_db* db, uint32_t first, uint32_t count,
              const bpx_launch* launch, uint32_t out_flags, bpx_result** out) {
-    if (!ctx || !q || !db || !out) return fail(BPX_E_ARG, "NULL handle");
-    *out = nullptr;
-    if (q->ctx != ctx || db->ctx != ctx) return fail(BPX_E_ARG, "handles belong to another context");
-    if ((uint64_t)first + count > db->n) return fail(BPX_E_ARG, "subject range outside the database");
-    HIP_TRY(hipSetDevice(ctx->device));
-    const bpx_launch L = resolve_launch(launch, q->table, q->qlen, diag_len_of(q->params), ctx->lds_limit);
-    const bpx::LdsPlan plan = bpx::plan_lds((uint32_t)q->table_bytes, q->qlen, diag_len_of(q->params),
-                                            (uint32_t)L.queue_capacity, L.waves_per_group,
-                                            (uint32_t)ctx->lds_limit, true);
-    if (plan.total > (uint64_t)ctx->lds_limit)
-        return fail(BPX_E_BUDGET, "LDS plan %u B exceeds the %lld B per workgroup (diag %d, %d waves)",
-                    plan.total, (long long)ctx->lds_limit, diag_len_of(q->params), L.waves_per_group);
+    if (!q) return fail(BPX_E_ARG, "query is NULL");
+    return start_scan(ctx, q->set, db, first, count, launch, out_flags, out);
+}
 
-    auto* r = new (std::nothrow) bpx_result;
-    if (!r) return fail(BPX_E_NOMEM, "host allocation failed");
-    r->ctx = ctx;
-    r->first = first;
-    r->count = count;
-    r->flags = out_flags;
-    r->q = q;
-    r->db = db;
-    r->launch = L;
-    for (auto& ev : r->ev)
-        if (hipEventCreate(&ev) != hipSuccess) {
-            bpx_result_destroy(r);
-            return fail(BPX_E_HIP, "event creation failed");
-        }
-    if (hipHostMalloc((void**)&r->h_ustats, 8 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
-        bpx_result_destroy(r);
-        return fail(BPX_E_NOMEM, "pinned allocation failed");
-    }
-    const uint32_t waves = (uint32_t)(ctx->num_cus * L.groups_per_cu * L.waves_per_group);
-    const uint32_t seg_cap = std::max<uint32_t>(64, (uint32_t)(((uint64_t)count / 2 + 65536) / waves + 1));
-    int rc = alloc_result_buffers(r, seg_cap, waves);
-    if (rc) {
-        bpx_result_destroy(r);
-        return rc;
-    }
-    const uint32_t* order = db->d_order;
-    if (!(first == 0 && count == db->n)) {
-        std::vector<uint32_t> sub;
-        sub.reserve(count);
-        for (uint32_t id : db->h_order)
-            if (id >= first && id - first < count) sub.push_back(id);
-        r->d_order = (uint32_t*)ctx->pool.get(std::max<size_t>(count, 1) * 4);
-        if (!r->d_order) {
-            bpx_result_destroy(r);
-            return fail(BPX_E_NOMEM, "work list allocation failed");
-        }
-        if (count) {
-            hipError_t e = hipMemcpyAsync(r->d_order, sub.data(), (size_t)count * 4, hipMemcpyHostToDevice,
-                                          ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) {
-                bpx_result_destroy(r);
-                return fail(BPX_E_HIP, "work list upload failed: %s", hipGetErrorString(e));
-            }
-        }
-        order = r->d_order;
-    }
-    rc = enqueue_scan(r, q, db, L, plan, order);
-    if (rc) {
-        bpx_result_destroy(r);
-        return rc;
-    }
-    *out = r;
-    return BPX_OK;
+int bpx_scan_batch(bpx_ctx* ctx, const bpx_batch* b, const bpx_db* db, uint32_t first, uint32_t count,
+                   const bpx_launch* launch, uint32_t out_flags, bpx_result** out) {
+    if (!b) return fail(BPX_E_ARG, "batch is NULL");
+    return start_scan(ctx, b->set, db, first, count, launch, out_flags, out);
 }
 
 int bpx_result_wait(bpx_result* r) {
@@ -624,39 +933,30 @@ int bpx_result_wait(bpx_result* r) {
     HIP_TRY(hipSetDevice(ctx->device));
     for (;;) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        const uint32_t ovf = (uint32_t)(r->h_ustats[6] & 0xFFFFFFFFu);
+        const uint32_t ovf = (uint32_t)(r->h_meta[1] & 0xFFFFFFFFu);
         if (ovf & 2u) return fail(BPX_E_STATE, "internal: hit ring smaller than a lookup cell");
         if (!(ovf & 1u)) break;
         // a wave's record segment filled: rescan with 4x the segment size
-        const uint32_t cap = r->seg_cap * 4, waves = r->seg_waves;
+        const uint32_t cap = r->seg_cap * 4;
         const uint32_t* order = r->d_order ? r->d_order : r->db->d_order;
-        uint32_t* keep_order = r->d_order;
-        r->d_order = nullptr;
-        result_release(r);
-        r->d_order = keep_order;
-        int rc = alloc_result_buffers(r, cap, waves);
+        result_release(r, true);
+        int rc = alloc_buffers(r, cap);
         if (rc) return rc;
-        const bpx::LdsPlan plan = bpx::plan_lds((uint32_t)r->q->table_bytes, r->q->qlen,
-                                                diag_len_of(r->q->params), (uint32_t)r->launch.queue_capacity,
-                                                r->launch.waves_per_group, (uint32_t)ctx->lds_limit, true);
-        rc = enqueue_scan(r, r->q, r->db, r->launch, plan, order);
+        rc = enqueue(r, order);
         if (rc) return rc;
         ++r->reruns;
     }
-    r->total = r->h_ustats[5];
-    r->ustats.lookup_hits = (int64_t)r->h_ustats[0];
-    r->ustats.num_seqs_lookup_hits = (int64_t)r->h_ustats[1];
-    r->ustats.init_extends = (int64_t)r->h_ustats[2];
-    r->ustats.good_init_extends = (int64_t)r->h_ustats[3];
-    r->ustats.num_seqs_passed = (int64_t)r->h_ustats[4];
+    r->total = r->h_meta[0];
     if (r->total) {
         r->h_records = (bpx_init_hsp*)ctx->pinned.get(r->total * sizeof(bpx_init_hsp));
-        r->h_subjects = (uint32_t*)ctx->pinned.get(r->total * sizeof(uint32_t));
-        if (!r->h_records || !r->h_subjects) return fail(BPX_E_NOMEM, "pinned allocation for records failed");
-        HIP_TRY(hipMemcpyAsync(r->h_records, r->d_records, r->total * sizeof(bpx_init_hsp),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(r->h_subjects, r->d_subjects, r->total * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, ctx->stream));
+        r->h_subjects = (uint32_t*)ctx->pinned.get(r->total * 4);
+        r->h_queries = (uint32_t*)ctx->pinned.get(r->total * 4);
+        if (!r->h_records || !r->h_subjects || !r->h_queries)
+            return fail(BPX_E_NOMEM, "pinned allocation for records failed");
+        HIP_TRY(hipMemcpyAsync(r->h_records, r->d_records, r->total * sizeof(bpx_init_hsp), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipMemcpyAsync(r->h_subjects, r->d_subjects, r->total * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(r->h_queries, r->d_queries, r->total * 4, hipMemcpyDeviceToHost, ctx->stream));
     }
     HIP_TRY(hipEventRecord(r->ev[2], ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -674,30 +974,61 @@ int bpx_result_timing(const bpx_result* r, float* kernel_ms, float* device_ms) {
     return BPX_OK;
 }
 
+int bpx_result_query_stats(const bpx_result* r, bpx_ungapped_stats* stats, uint32_t capacity) {
+    if (!r) return fail(BPX_E_ARG, "result is NULL");
+    if (!r->waited) return fail(BPX_E_STATE, "result not waited");
+    if (capacity < r->set->nq) return fail(BPX_E_ARG, "capacity %u < %u queries", capacity, r->set->nq);
+    for (uint32_t i = 0; i < r->set->nq && stats; ++i) {
+        const unsigned long long* u = r->h_meta + kMetaWords + 5ull * i;
+        stats[i].lookup_hits = (int64_t)u[0];
+        stats[i].num_seqs_lookup_hits = (int64_t)u[1];
+        stats[i].init_extends = (int64_t)u[2];
+        stats[i].good_init_extends = (int64_t)u[3];
+        stats[i].num_seqs_passed = (int64_t)u[4];
+    }
+    return BPX_OK;
+}
+
 int bpx_result_summary(const bpx_result* r, uint64_t* num_records, bpx_ungapped_stats* stats) {
     if (!r) return fail(BPX_E_ARG, "result is NULL");
     if (!r->waited) return fail(BPX_E_STATE, "result not waited");
     if (num_records) *num_records = r->total;
-    if (stats) *stats = r->ustats;
+    if (stats) {
+        bpx_ungapped_stats s{};
+        for (uint32_t i = 0; i < r->set->nq; ++i) {
+            const unsigned long long* u = r->h_meta + kMetaWords + 5ull * i;
+            s.lookup_hits += (int64_t)u[0];
+            s.num_seqs_lookup_hits += (int64_t)u[1];
+            s.init_extends += (int64_t)u[2];
+            s.good_init_extends += (int64_t)u[3];
+            s.num_seqs_passed += (int64_t)u[4];
+        }
+        *stats = s;
+    }
     return BPX_OK;
 }
 
-int bpx_result_records(const bpx_result* r, bpx_init_hsp* records, uint32_t* subjects, uint64_t capacity) {
+int bpx_result_records(const bpx_result* r, bpx_init_hsp* records, uint32_t* subjects, uint32_t* queries,
+                       uint64_t capacity) {
     if (!r) return fail(BPX_E_ARG, "result is NULL");
     if (!r->waited) return fail(BPX_E_STATE, "result not waited");
-    if (capacity < r->total) return fail(BPX_E_ARG, "capacity %llu < %llu records", (unsigned long long)capacity,
-                                         (unsigned long long)r->total);
+    if (capacity < r->total)
+        return fail(BPX_E_ARG, "capacity %llu < %llu records", (unsigned long long)capacity,
+                    (unsigned long long)r->total);
     if (records && r->total) std::memcpy(records, r->h_records, r->total * sizeof(bpx_init_hsp));
-    if (subjects && r->total) std::memcpy(subjects, r->h_subjects, r->total * sizeof(uint32_t));
+    if (subjects && r->total) std::memcpy(subjects, r->h_subjects, r->total * 4);
+    if (queries && r->total) std::memcpy(queries, r->h_queries, r->total * 4);
     return BPX_OK;
 }
 
-int bpx_result_subject_stats(const bpx_result* r, bpx_subject_stats* stats, uint32_t capacity) {
+int bpx_result_subject_stats(const bpx_result* r, bpx_subject_stats* stats, uint64_t capacity) {
     if (!r) return fail(BPX_E_ARG, "result is NULL");
     if (!r->waited) return fail(BPX_E_STATE, "result not waited");
     if (!(r->flags & BPX_OUT_SUBJECT_STATS)) return fail(BPX_E_STATE, "scan made without BPX_OUT_SUBJECT_STATS");
-    if (capacity < r->count) return fail(BPX_E_ARG, "capacity %u < %u subjects", capacity, r->count);
-    if (stats && r->count) std::memcpy(stats, r->h_sstats, (size_t)r->count * sizeof(bpx_subject_stats));
+    const uint64_t pairs = (uint64_t)r->set->nq * r->count;
+    if (capacity < pairs)
+        return fail(BPX_E_ARG, "capacity %llu < %llu entries", (unsigned long long)capacity, (unsigned long long)pairs);
+    if (stats && pairs) std::memcpy(stats, r->h_sstats, pairs * sizeof(bpx_subject_stats));
     return BPX_OK;
 }
 
@@ -705,8 +1036,7 @@ int bpx_result_destroy(bpx_result* r) {
     if (!r) return BPX_OK;
     (void)hipSetDevice(r->ctx->device);
     (void)hipStreamSynchronize(r->ctx->stream);
-    result_release(r);
-    if (r->h_ustats) (void)hipHostFree(r->h_ustats);
+    result_release(r, false);
     for (auto& ev : r->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete r;

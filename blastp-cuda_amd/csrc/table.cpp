@@ -54,10 +54,22 @@ void enumerate_neighbours(const uint8_t* qw, int word_length, const int8_t* M,
 
 bool build_table(const uint8_t* query, int32_t qlen, const int8_t* M,
                  int32_t threshold, int32_t word_length, HostTable* out) {
-    if (!out || !M || word_length < 2 || word_length > 3 || qlen < 0 || qlen > 65535) return false;
-    if (qlen > 0 && !query) return false;
-    for (int32_t i = 0; i < qlen; ++i)
-        if (query[i] >= kAlphabet) return false;
+    QuerySpan one{query, qlen};
+    return build_table_multi(&one, 1, M, threshold, word_length, out);
+}
+
+bool build_table_multi(const QuerySpan* queries, int32_t nq, const int8_t* M,
+                       int32_t threshold, int32_t word_length, HostTable* out) {
+    if (!out || !M || word_length < 2 || word_length > 3 || nq < 0 || (nq > 0 && !queries)) return false;
+    int64_t total = 0;
+    for (int32_t k = 0; k < nq; ++k) {
+        const QuerySpan& q = queries[k];
+        if (q.len < 0 || (q.len > 0 && !q.seq)) return false;
+        for (int32_t i = 0; i < q.len; ++i)
+            if (q.seq[i] >= kAlphabet) return false;
+        total += q.len;
+    }
+    if (total > 65535) return false;
 
     HostTable t;
     t.word_length = word_length;
@@ -71,19 +83,24 @@ bool build_table(const uint8_t* query, int32_t qlen, const int8_t* M,
         row_max[i] = m;
     }
 
-    // Distinct query words -> ascending positions.
-    const int32_t npos = qlen - word_length + 1;
+    // Distinct query words -> ascending positions in the concatenation.
     std::unordered_map<uint32_t, std::vector<uint16_t>> by_word;
     std::vector<uint32_t> word_order;  // first-occurrence order, for determinism
-    for (int32_t p = 0; p < npos; ++p) {
-        const uint32_t w = word_index(query + p, word_length);
-        auto it = by_word.find(w);
-        if (it == by_word.end()) {
-            word_order.push_back(w);
-            by_word.emplace(w, std::vector<uint16_t>{(uint16_t)p});
-        } else {
-            it->second.push_back((uint16_t)p);
+    int32_t base = 0;
+    for (int32_t k = 0; k < nq; ++k) {
+        const QuerySpan& q = queries[k];
+        const int32_t npos = q.len - word_length + 1;
+        for (int32_t p = 0; p < npos; ++p) {
+            const uint32_t w = word_index(q.seq + p, word_length);
+            auto it = by_word.find(w);
+            if (it == by_word.end()) {
+                word_order.push_back(w);
+                by_word.emplace(w, std::vector<uint16_t>{(uint16_t)(base + p)});
+            } else {
+                it->second.push_back((uint16_t)(base + p));
+            }
         }
+        base += q.len;
     }
 
     // Count, then fill, per neighbour word.

@@ -49,4 +49,15 @@ struct HostTable {
 bool build_table(const uint8_t* query, int32_t qlen, const int8_t* matrix28,
                  int32_t threshold, int32_t word_length, HostTable* out);
 
+// One table for several queries scanned together: query k occupies
+// positions [base_k, base_k + len_k) of their concatenation and its words
+// are listed under base_k + p.  Words never span two queries.  The
+// concatenation must be shorter than 65,536 residues.
+struct QuerySpan {
+    const uint8_t* seq;
+    int32_t len;
+};
+bool build_table_multi(const QuerySpan* queries, int32_t nq, const int8_t* matrix28,
+                       int32_t threshold, int32_t word_length, HostTable* out);
+
 }  // namespace bpx

@@ -15,6 +15,10 @@
  *   bpx_query_prepare    lookup/PV/matrix/query staging gpu_blastp.cu:45-88, 140-145, 159-163, 177-183
  *   bpx_check_memory     GPU_BLASTP_check_memory        gpu_cpu_common.h:81-89, gpu_blastp.cu:282-334
  *   bpx_scan             GPU_BLASTP (launch)            gpu_cpu_common.h:44-63, gpu_blastp.cu:16-222
+ *   bpx_batch_*, bpx_scan_batch: several queries per launch -- the reference's
+ *                        multi-context launch (num_queries contexts,
+ *                        gpu_blastp_kernel.cu:238-241), with per-query
+ *                        diagonal state
  *   bpx_result_wait      GPU_BLASTP_get_data (sync)     gpu_cpu_common.h:65-69, gpu_blastp.cu:224-259
  *   bpx_result_*         h_Hits / h_GPUBlastInitHitList consumed at blast_engine.c:520-559
  *   bpx_*_destroy        GPU_BLASTP_free_memory         gpu_cpu_common.h:71-79, gpu_blastp.cu:262-280
@@ -101,9 +105,20 @@ typedef struct bpx_ungapped_stats {
 typedef struct bpx_launch {
     int32_t waves_per_group;    /* wavefronts per workgroup (1..16) */
     int32_t groups_per_cu;      /* resident workgroups per CU (grid = CUs x this) */
-    int32_t subjects_per_grab;  /* work items claimed per queue pop */
-    int32_t queue_capacity;     /* per-wave LDS hit queue entries (power of two) */
+    int32_t subjects_per_item;  /* subjects per dynamic work item */
+    int32_t queue_capacity;     /* per-wave LDS hit ring entries (power of two) */
+    int32_t subject_buffer;     /* per-wave LDS subject buffer bytes (power of two >= 1024) */
+    int32_t reserved;
 } bpx_launch;
+
+/* How a batch packs its queries into scan groups: queries shorter than
+ * group_residues are concatenated (at most group_queries per group) and share
+ * one lookup table and one pass over every subject; longer queries form
+ * groups of one.  Results do not depend on the packing. */
+typedef struct bpx_batch_policy {
+    int32_t group_residues;   /* 0 = default */
+    int32_t group_queries;    /* 0 = default, at most 255 */
+} bpx_batch_policy;
 
 /* Output selection for a scan. */
 #define BPX_OUT_SUBJECT_STATS 1u /* copy per-subject counters to the host */
@@ -143,29 +158,54 @@ int bpx_check_memory(bpx_ctx *ctx, const bpx_db *db, const bpx_query *q,
                      uint32_t first, uint32_t count, const bpx_launch *launch,
                      size_t *device_bytes, size_t *lds_bytes);
 
+/* ---- batch: many queries scanned in one persistent launch ---------------
+ * queries[i] has qlens[i] residues.  params gives word_length, threshold,
+ * window (shared by the batch) and the default x_dropoff / cutoff_score;
+ * x_dropoff / cutoff (nullable) override them per query (per NCBI context,
+ * gpu_blastp.cu:77-82).  params->diag_mask must be 0 (derived per query). */
+int bpx_batch_prepare(bpx_ctx *ctx, uint32_t num_queries, const uint8_t *const *queries,
+                      const int32_t *qlens, const int8_t *matrix,
+                      const bpx_word_params *params, const int32_t *x_dropoff,
+                      const int32_t *cutoff, const bpx_batch_policy *policy,
+                      bpx_batch **out);
+int bpx_batch_info(const bpx_batch *b, uint32_t *num_queries, uint32_t *num_groups,
+                   int64_t *table_bytes);
+int bpx_batch_destroy(bpx_batch *b);
+
 /* ---- scan ---------------------------------------------------------------
  * Asynchronous: enqueues the scan of subjects [first, first+count) on the
  * context's stream and returns.  launch may be NULL (defaults). */
 int bpx_scan(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, uint32_t first,
              uint32_t count, const bpx_launch *launch, uint32_t out_flags,
              bpx_result **out);
+/* The same for every query of a batch, in one persistent launch per table
+ * placement.  Results are identical to scanning each query alone. */
+int bpx_scan_batch(bpx_ctx *ctx, const bpx_batch *b, const bpx_db *db, uint32_t first,
+                   uint32_t count, const bpx_launch *launch, uint32_t out_flags,
+                   bpx_result **out);
 /* Blocks until the scan and its result copies are complete. */
 int bpx_result_wait(bpx_result *r);
-/* Kernel time of the last scan, measured with HIP events on the stream the
- * kernels run on: the word-finder kernel alone, and the whole device phase
+/* Times of the last scan, measured with HIP events on the stream the
+ * kernels run on: the word-finder kernels alone, and the whole device phase
  * (word finder + compaction + copies to the host). */
 int bpx_result_timing(const bpx_result *r, float *kernel_ms, float *device_ms);
-/* Number of records (all subjects) and the query-level ungapped stats. */
+/* Number of records (all queries and subjects) and the ungapped stats
+ * summed over the scan's queries. */
 int bpx_result_summary(const bpx_result *r, uint64_t *num_records,
                        bpx_ungapped_stats *stats);
-/* Records, ordered by (subject, subject_offset, query_offset); subjects[i]
- * is the absolute subject index of records[i].  Either pointer may be NULL. */
-int bpx_result_records(const bpx_result *r, bpx_init_hsp *records,
-                       uint32_t *subjects, uint64_t capacity);
-/* Per-subject counters for the scanned range (count entries); available
- * when the scan was made with BPX_OUT_SUBJECT_STATS. */
+/* Per-query ungapped stats (num_queries entries, in batch order). */
+int bpx_result_query_stats(const bpx_result *r, bpx_ungapped_stats *stats,
+                           uint32_t capacity);
+/* Records, ordered by (query, subject, subject_offset, query_offset) -- for
+ * one query and subject that is the sequential emission order; subjects[i]
+ * and queries[i] are the absolute subject index and the batch query index of
+ * records[i].  Any output pointer may be NULL. */
+int bpx_result_records(const bpx_result *r, bpx_init_hsp *records, uint32_t *subjects,
+                       uint32_t *queries, uint64_t capacity);
+/* Per-(query, subject) counters, query-major (num_queries x count entries);
+ * available when the scan was made with BPX_OUT_SUBJECT_STATS. */
 int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
-                             uint32_t capacity);
+                             uint64_t capacity);
 int bpx_result_destroy(bpx_result *r);
 
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */

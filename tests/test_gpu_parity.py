@@ -226,8 +226,9 @@ def test_segment_overflow_rescan(ctx):
 
 
 @pytest.mark.parametrize("launch", [dict(waves_per_group=1), dict(waves_per_group=4, groups_per_cu=2),
-                                    dict(waves_per_group=16, subjects_per_grab=8),
-                                    dict(queue_capacity=4096)])
+                                    dict(waves_per_group=16, subjects_per_item=8),
+                                    dict(queue_capacity=4096), dict(subject_buffer=1024),
+                                    dict(subject_buffer=8192, waves_per_group=4)])
 def test_launch_shapes_do_not_change_results(ctx, launch):
     rng = np.random.default_rng(10)
     q = bpx.synth_query(3, 0, 350)
@@ -267,3 +268,52 @@ def test_config2_full(ctx):
     orc = oracle_run(q, db.residues, db.offsets, db.lengths)
     say(f"config 2: oracle done, {len(orc[2])} records")
     assert_parity(gpu, orc)
+
+
+def batch_db(queries, n=3000, seed=31):
+    spec = dict(seed=seed, num_subjects=n, len_median=300, len_sigma=0.8, len_min=20, len_max=6000,
+                homolog_fraction=0.05, identity_min=0.4, identity_max=0.7, indel_min=0.0, indel_max=0.03,
+                biased_fraction=0.1, dirichlet_alpha=0.3, motif_min=5, motif_max=30)
+    return bpx.synth_db(spec, [q for q in queries if len(q) > 0], nthreads=THREADS)
+
+
+def assert_batch_parity(ctx, queries, db, xdrop=None, cutoff=None, **policy):
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    b = ctx.prepare_batch(queries, x_dropoff=xdrop, cutoff=cutoff, **policy)
+    say(f"batch: {b.info()}")
+    r = bpx.scan_batch(ctx, b, ddb, subject_stats=True).wait()
+    recs, subs, qids = r.records(with_queries=True)
+    st = r.subject_stats().reshape(len(queries), db.num_subjects)
+    qstats = r.query_stats()
+    assert (np.diff(qids.astype(np.int64)) >= 0).all()
+    for i, q in enumerate(queries):
+        prm = O.params(len(q), x_dropoff=int(xdrop[i]) if xdrop is not None else 16,
+                       cutoff=int(cutoff[i]) if cutoff is not None else 41)
+        orc = O.run_db(q, db.residues, db.offsets, db.lengths, prm, THREADS)
+        sel = qids == i
+        gpu = (st[i], recs[sel], subs[sel], qstats[i], None)
+        assert_parity(gpu, orc)
+    r.close()
+    b.close()
+    ddb.close()
+
+
+def test_batch_equals_single_queries(ctx):
+    rng = np.random.default_rng(21)
+    lens = [int(x) for x in rng.integers(5, 900, 40)] + [2000, 3, 0, 1]
+    queries = [bpx.synth_query(21, i, L) for i, L in enumerate(lens)]
+    db = batch_db(queries)
+    assert_batch_parity(ctx, queries, db)
+
+
+def test_batch_per_query_parameters_and_policies(ctx):
+    rng = np.random.default_rng(22)
+    lens = [int(x) for x in rng.integers(20, 400, 30)]
+    queries = [bpx.synth_query(22, i, L) for i, L in enumerate(lens)]
+    db = batch_db(queries, n=1500, seed=32)
+    xd = rng.integers(8, 30, len(queries)).astype(np.int32)
+    co = rng.integers(25, 50, len(queries)).astype(np.int32)
+    for pol in (dict(group_residues=300, group_queries=4), dict(group_residues=4000, group_queries=200),
+                dict(group_residues=1)):
+        say(f"policy {pol}")
+        assert_batch_parity(ctx, queries, db, xd, co, **pol)
