@@ -84,13 +84,14 @@ struct TwoHit {
 };
 
 // Subject residue p of the wave's staged subject.  RESIDENT: the whole
-// subject is in sbuf; otherwise sbuf is a window [whi - sbuf, whi) and other
-// positions come from HBM.
+// subject is in sbuf; otherwise sbuf holds the window [wlo, whi) and every
+// other position (behind the window, or ahead of it for a right extension
+// that runs past the loaded part) comes from HBM.
 template <bool RESIDENT>
 __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
-                                         int32_t wlo, int32_t slen) {
-    if (RESIDENT || p >= wlo) return sbuf[p & smask];
-    return p < slen ? S[p] : 0u;
+                                         int32_t wlo, int32_t whi, int32_t slen) {
+    if (RESIDENT || (p >= wlo && p < whi)) return sbuf[p & smask];
+    return (p >= 0 && p < slen) ? S[p] : 0u;
 }
 
 // s_BlastAaExtendTwoHit semantics (gpu_blastp_kernel.cu:341-415 with the
@@ -101,11 +102,12 @@ __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, co
 template <int WORD, bool RESIDENT>
 __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
                                                  const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t wlo,
-                                                 int32_t slen, int32_t s_left, int32_t s, int32_t c, int32_t xdrop) {
+                                                 int32_t whi, int32_t slen, int32_t s_left, int32_t s, int32_t c,
+                                                 int32_t xdrop) {
     int32_t sc = 0, best = 0, rd = 0;
 #pragma unroll
     for (int32_t i = 0; i < WORD; ++i) {
-        sc += M[Q[c + i] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, s + i, wlo, slen)];
+        sc += M[Q[c + i] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, s + i, wlo, whi, slen)];
         if (sc > best) {
             best = sc;
             rd = i + 1;
@@ -115,7 +117,7 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     const int32_t n = min(sr - 1, qr - 1);
     int32_t score = 0, mx = 0, left_len = 0;
     for (int32_t k = 0; k <= n; ++k) {
-        score += M[Q[cr - 1 - k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr - 1 - k, wlo, slen)];
+        score += M[Q[cr - 1 - k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr - 1 - k, wlo, whi, slen)];
         if (score > mx) {
             mx = score;
             left_len = k + 1;
@@ -130,7 +132,7 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
         const int32_t n2 = min(slen - sr, qlen - qr);
         int32_t rscore = mx, rmax = mx, best_k = -1, k = 0;
         for (; k < n2; ++k) {
-            rscore += M[Q[cr + k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr + k, wlo, slen)];
+            rscore += M[Q[cr + k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr + k, wlo, whi, slen)];
             if (rscore > rmax) {
                 rmax = rscore;
                 best_k = k;
@@ -279,10 +281,10 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                     const int32_t wlo = whi - (int32_t)p.sbuf;
                     if (resident)
                         x = extend_two_hit<WORD, true>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
-                                                       wlo, slen, sl, s, c, qi.xdrop);
+                                                       wlo, whi, slen, sl, s, c, qi.xdrop);
                     else
                         x = extend_two_hit<WORD, false>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
-                                                        wlo, slen, sl, s, c, qi.xdrop);
+                                                        wlo, whi, slen, sl, s, c, qi.xdrop);
                     rec = x.score >= qi.cutoff;
                     diag[dslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
                                                  : (uint32_t)(s + doff);
