@@ -216,6 +216,7 @@ struct bpx_result {
     std::vector<Class> classes;
     std::vector<bpx::WorkItem> items;
     uint32_t qcap = 0;
+    uint32_t lds_need = 0;  // largest per-workgroup LDS plan, including one that did not fit
 };
 
 namespace {
@@ -401,6 +402,7 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_laun
             waves /= 2;
             pl = plan_of(a, c.tab_lds, waves);
         }
+        r->lds_need = std::max(r->lds_need, pl.total);
         if (pl.total > limit)
             return fail(BPX_E_BUDGET,
                         "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
@@ -900,8 +902,7 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
     bpx_result tmp;
     tmp.ctx = ctx;
     int rc = plan_scan(&tmp, q->set, count, launch);
-    uint32_t lds = 0;
-    for (const auto& c : tmp.classes) lds = std::max(lds, c.plan.total);
+    const uint32_t lds = tmp.lds_need;
     const uint64_t waves = std::max<uint64_t>(total_waves(&tmp), 1);
     const uint64_t seg = std::max<uint64_t>(64, ((uint64_t)count / 8 + 65536) / waves + 1) * waves;
     const uint64_t need = db->arena_bytes + (uint64_t)db->n * 16 + q->set.table_bytes + (uint64_t)count * 24 +
