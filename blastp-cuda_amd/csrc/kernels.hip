@@ -83,15 +83,16 @@ struct TwoHit {
     bool right_extend;
 };
 
-// Subject residue p of the wave's staged subject.  RESIDENT: the whole
-// subject is in sbuf; otherwise sbuf holds the window [wlo, whi) and every
-// other position (behind the window, or ahead of it for a right extension
-// that runs past the loaded part) comes from HBM.
-template <bool RESIDENT>
+// Subject residue p of the wave's staged subject.  sbuf holds positions
+// [wlo, whi) (all of a subject that fits); any other position -- behind the
+// window, or ahead of it when a right extension runs past the loaded part --
+// comes from HBM.  The LDS read is unconditional so the common case stays a
+// plain ds_read.
 __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
                                          int32_t wlo, int32_t whi, int32_t slen) {
-    if (RESIDENT || (p >= wlo && p < whi)) return sbuf[p & smask];
-    return (p >= 0 && p < slen) ? S[p] : 0u;
+    uint32_t v = sbuf[p & smask];
+    if (p < wlo || p >= whi) v = (p >= 0 && p < slen) ? S[p] : 0u;
+    return v;
 }
 
 // s_BlastAaExtendTwoHit semantics (gpu_blastp_kernel.cu:341-415 with the
@@ -99,7 +100,14 @@ __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, co
 // the left from one past it, to the right only if the left reached the
 // first hit.  c is the second hit's position in the group's concatenated
 // residues Q; the query occupies Q[qstart, qstart + qlen).
-template <int WORD, bool RESIDENT>
+//
+// Both X-drop loops run in blocks of kBlock steps: the block's residues and
+// scores are loaded together, then the steps are applied in order in
+// registers with a `done` predicate, so each step still sees exactly the
+// sequential running score, maximum and stop test.
+constexpr int kBlock = 8;
+
+template <int WORD>
 __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
                                                  const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t wlo,
                                                  int32_t whi, int32_t slen, int32_t s_left, int32_t s, int32_t c,
@@ -107,40 +115,87 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     int32_t sc = 0, best = 0, rd = 0;
 #pragma unroll
     for (int32_t i = 0; i < WORD; ++i) {
-        sc += M[Q[c + i] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, s + i, wlo, whi, slen)];
+        sc += M[Q[c + i] * kMatrixStride + sres(sbuf, smask, S, s + i, wlo, whi, slen)];
         if (sc > best) {
             best = sc;
             rd = i + 1;
         }
     }
     const int32_t cr = c + rd, sr = s + rd, qr = cr - qstart;
+
+    // left: positions (cr-1-k, sr-1-k) for k = 0..n
     const int32_t n = min(sr - 1, qr - 1);
     int32_t score = 0, mx = 0, left_len = 0;
-    for (int32_t k = 0; k <= n; ++k) {
-        score += M[Q[cr - 1 - k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr - 1 - k, wlo, whi, slen)];
-        if (score > mx) {
-            mx = score;
-            left_len = k + 1;
+    bool done = n < 0;
+    for (int32_t k0 = 0; !done; k0 += kBlock) {
+        int32_t m[kBlock];
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) {
+            const int32_t k = k0 + i;
+            const bool ok = k <= n;
+            const uint32_t qb = ok ? Q[cr - 1 - k] : 0u;
+            const uint32_t sb = ok ? sres(sbuf, smask, S, sr - 1 - k, wlo, whi, slen) : 0u;
+            m[i] = ok ? (int32_t)M[qb * kMatrixStride + sb] : 0;
         }
-        if (mx - score >= xdrop) break;
+#pragma unroll
+        for (int i = 0; i < kBlock; ++i) {
+            const int32_t k = k0 + i;
+            if (!done) {
+                if (k > n) {
+                    done = true;
+                } else {
+                    score += m[i];
+                    if (score > mx) {
+                        mx = score;
+                        left_len = k + 1;
+                    }
+                    if (mx - score >= xdrop) done = true;
+                }
+            }
+        }
     }
     TwoHit r;
     r.right_extend = left_len >= sr - s_left;
     r.s_last = sr;
     int32_t right_len = 0, total = mx;
     if (r.right_extend) {
+        // right: positions (cr+k, sr+k) for k = 0..n2-1, from the left score
         const int32_t n2 = min(slen - sr, qlen - qr);
-        int32_t rscore = mx, rmax = mx, best_k = -1, k = 0;
-        for (; k < n2; ++k) {
-            rscore += M[Q[cr + k] * kMatrixStride + sres<RESIDENT>(sbuf, smask, S, sr + k, wlo, whi, slen)];
-            if (rscore > rmax) {
-                rmax = rscore;
-                best_k = k;
+        int32_t rscore = mx, rmax = mx, best_k = -1, kstop = n2;
+        bool rdone = n2 <= 0;
+        for (int32_t k0 = 0; !rdone; k0 += kBlock) {
+            int32_t m[kBlock];
+#pragma unroll
+            for (int i = 0; i < kBlock; ++i) {
+                const int32_t k = k0 + i;
+                const bool ok = k < n2;
+                const uint32_t qb = ok ? Q[cr + k] : 0u;
+                const uint32_t sb = ok ? sres(sbuf, smask, S, sr + k, wlo, whi, slen) : 0u;
+                m[i] = ok ? (int32_t)M[qb * kMatrixStride + sb] : 0;
             }
-            if (rscore <= 0 || rmax - rscore >= xdrop) break;
+#pragma unroll
+            for (int i = 0; i < kBlock; ++i) {
+                const int32_t k = k0 + i;
+                if (!rdone) {
+                    if (k >= n2) {
+                        rdone = true;
+                        kstop = n2;
+                    } else {
+                        rscore += m[i];
+                        if (rscore > rmax) {
+                            rmax = rscore;
+                            best_k = k;
+                        }
+                        if (rscore <= 0 || rmax - rscore >= xdrop) {
+                            rdone = true;
+                            kstop = k;
+                        }
+                    }
+                }
+            }
         }
         right_len = best_k + 1;
-        r.s_last = sr + k;
+        r.s_last = sr + kstop;
         total = rmax;
     }
     r.hsp_q = qr - left_len;
@@ -158,7 +213,7 @@ struct QCount {
 };
 
 template <int WORD, bool TAB_LDS, bool MULTI>
-__global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
+__global__ __launch_bounds__(512) void k_scan(ScanParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -208,7 +263,12 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
         }
         __syncthreads();
         const uint32_t item = ctl[0];
-        if (item >= p.nitems) break;
+        if (item >= p.nitems) {
+            // largest record count any wave produced: sizes a rescan after
+            // a segment overflow exactly
+            if (lane == 0 && seg_fill > p.seg_cap) atomicMax(p.max_fill, seg_fill);
+            break;
+        }
         const WorkItem w = p.items[item];
         if (w.group != cur_group) {
             g = p.groups[w.group];
@@ -243,14 +303,13 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
             const uint8_t* __restrict__ S = p.residues + p.soff[sid];
             const int32_t slen = (int32_t)p.slen[sid];
             const int32_t send = (slen + 15) & ~15;
-            const bool resident = slen <= (int32_t)p.sbuf;
 
             // ---- stage the subject (or its first window) into LDS ----
             auto fill = [&](int32_t from, int32_t to) {
                 for (int32_t x = from + 16 * (int32_t)lane; x < to; x += 1024)
                     *reinterpret_cast<uint4*>(sbuf + (x & smask)) = *reinterpret_cast<const uint4*>(S + x);
             };
-            int32_t whi = resident ? send : min(send, (int32_t)p.sbuf);
+            int32_t whi = min(send, (int32_t)p.sbuf);
             fill(0, whi);
             wave_sync();
 
@@ -259,72 +318,12 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
             uint32_t head = 0, count = 0, ntasks = 0;     // wave-uniform
             int32_t qbase = 0;
 
-            // Runs the queued extensions, one per lane, and applies their
-            // outcome to the diagonal table.
-            auto flush_tasks = [&]() {
-                const uint32_t nt = ntasks;
-                if (nt == 0) return;
-                wave_sync();
-                const bool mine = lane < nt;
-                bool rec = false;
-                TwoHit x{};
-                uint32_t j = 0, dslot = 0;
-                int32_t s = 0, c = 0;
-                if (mine) {
-                    dslot = t_slot[lane];
-                    const uint32_t cj = t_cj[lane];
-                    c = (int32_t)(cj >> 16);
-                    j = cj & 0xFFFFu;
-                    s = (int32_t)t_s[lane];
-                    const int32_t sl = (int32_t)t_sl[lane];
-                    const QueryInfo qi = MULTI ? qinfo[j] : q0;
-                    const int32_t wlo = whi - (int32_t)p.sbuf;
-                    if (resident)
-                        x = extend_two_hit<WORD, true>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
-                                                       wlo, whi, slen, sl, s, c, qi.xdrop);
-                    else
-                        x = extend_two_hit<WORD, false>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
-                                                        wlo, whi, slen, sl, s, c, qi.xdrop);
-                    rec = x.score >= qi.cutoff;
-                    diag[dslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
-                                                 : (uint32_t)(s + doff);
-                    if (MULTI) {
-                        atomicAdd(&qc[j].ext, 1u);
-                        if (rec) atomicAdd(&qc[j].saved, 1u);
-                    }
-                    c -= (int32_t)qi.qstart;  // query-local offset of the second hit
-                }
-                const unsigned long long rb = __ballot(rec);
-                if (!MULTI) {
-                    extended += nt;
-                    saved += (uint32_t)__popcll(rb);
-                }
-                if (rb) {
-                    const uint32_t at = seg_fill + rank_below(rb);
-                    if (rec) {
-                        if (at < p.seg_cap) {
-                            SegRecord r;
-                            r.h.hsp_q = x.hsp_q;
-                            r.h.hsp_s = x.hsp_s;
-                            r.h.query_offset = c;
-                            r.h.subject_offset = s;
-                            r.h.hsp_len = x.len;
-                            r.h.score = x.score;
-                            r.j = j;
-                            r.pad = 0;
-                            p.seg[seg_base + at] = r;
-                        } else {
-                            atomicOr(p.overflow, 1u);
-                        }
-                    }
-                    seg_fill += (uint32_t)__popcll(rb);
-                }
-                ntasks = 0;
-                wave_sync();
-            };
-
-            // Applies the oldest n (<= 64) ring entries to the diagonal table.
-            auto drain_batch = [&](uint32_t n) {
+            // Applies the oldest n (<= 64) ring entries to the diagonal
+            // table.  Completed pairs are queued; the queue is flushed -- the
+            // extensions run one per lane and their outcome is written back --
+            // when a hit lands on a pending slot, when the queue would
+            // overflow, and (force) after the last batch of the subject.
+            auto drain_batch = [&](uint32_t n, bool force) {
                 bool active = lane < n;
                 const uint32_t e = active ? ring[(head + lane) & qmask] : 0u;
                 const int32_t c = (int32_t)(e >> 16);
@@ -343,46 +342,111 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
                 const uint32_t h = dslot & (kClaimSlots - 1);
-                while (__ballot(active)) {
-                    if (active) atomicMin(&claim[h], lane);
-                    wave_sync();
-                    const bool win = active && claim[h] == lane;
-                    wave_sync();
+                bool force_left = force;
+                for (;;) {
+                    const bool any = __ballot(active) != 0;
+                    if (!any && !(force_left && ntasks > 0)) break;
                     bool conflict = false, task = false, write = false;
                     uint32_t ns = 0;
                     int32_t sl = 0;
-                    if (win) {
-                        claim[h] = 0xFFFFFFFFu;
-                        const uint32_t st = diag[dslot];
-                        if (st & kPend) {
-                            conflict = true;
-                        } else {
-                            active = false;
-                            const int32_t v = (int32_t)(st & kVal);
-                            if (st & kFlag) {
-                                if (s + doff >= v) {
-                                    ns = (uint32_t)(s + doff);
-                                    write = true;
-                                }
+                    bool need_flush;
+                    unsigned long long tb = 0;
+                    if (any) {
+                        if (active) atomicMin(&claim[h], lane);
+                        wave_sync();
+                        const bool win = active && claim[h] == lane;
+                        wave_sync();
+                        if (win) {
+                            claim[h] = 0xFFFFFFFFu;
+                            const uint32_t st = diag[dslot];
+                            if (st & kPend) {
+                                conflict = true;
                             } else {
-                                const int32_t lh = v - doff;
-                                const int32_t diff = s - lh;
-                                if (diff >= window) {
-                                    ns = (uint32_t)(s + doff);
-                                    write = true;
-                                } else if (diff >= WORD) {
-                                    task = true;
-                                    sl = lh + WORD;
-                                    ns = kPend;
-                                    write = true;
+                                active = false;
+                                const int32_t v = (int32_t)(st & kVal);
+                                if (st & kFlag) {
+                                    if (s + doff >= v) {
+                                        ns = (uint32_t)(s + doff);
+                                        write = true;
+                                    }
+                                } else {
+                                    const int32_t lh = v - doff;
+                                    const int32_t diff = s - lh;
+                                    if (diff >= window) {
+                                        ns = (uint32_t)(s + doff);
+                                        write = true;
+                                    } else if (diff >= WORD) {
+                                        task = true;
+                                        sl = lh + WORD;
+                                        ns = kPend;
+                                        write = true;
+                                    }
                                 }
                             }
                         }
+                        tb = __ballot(task);
+                        need_flush = (ntasks + (uint32_t)__popcll(tb) > kTaskCap) || __ballot(conflict) != 0;
+                    } else {
+                        need_flush = true;
+                        force_left = false;
                     }
-                    const unsigned long long tb = __ballot(task);
+                    if (need_flush && ntasks > 0) {
+                        // ---- run the queued extensions ----
+                        const uint32_t nt = ntasks;
+                        wave_sync();
+                        const bool mine = lane < nt;
+                        bool rec = false;
+                        TwoHit x{};
+                        uint32_t tj = 0;
+                        int32_t ts = 0, tc = 0;
+                        if (mine) {
+                            const uint32_t tslot = t_slot[lane];
+                            const uint32_t cj = t_cj[lane];
+                            tc = (int32_t)(cj >> 16);
+                            tj = cj & 0xFFFFu;
+                            ts = (int32_t)t_s[lane];
+                            const int32_t tsl = (int32_t)t_sl[lane];
+                            const QueryInfo qi = MULTI ? qinfo[tj] : q0;
+                            x = extend_two_hit<WORD>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
+                                                     whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc, qi.xdrop);
+                            rec = x.score >= qi.cutoff;
+                            diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
+                                                         : (uint32_t)(ts + doff);
+                            if (MULTI) {
+                                atomicAdd(&qc[tj].ext, 1u);
+                                if (rec) atomicAdd(&qc[tj].saved, 1u);
+                            }
+                            tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
+                        }
+                        const unsigned long long rb = __ballot(rec);
+                        if (!MULTI) {
+                            extended += nt;
+                            saved += (uint32_t)__popcll(rb);
+                        }
+                        if (rb) {
+                            const uint32_t at = seg_fill + rank_below(rb);
+                            if (rec) {
+                                if (at < p.seg_cap) {
+                                    SegRecord r;
+                                    r.h.hsp_q = x.hsp_q;
+                                    r.h.hsp_s = x.hsp_s;
+                                    r.h.query_offset = tc;
+                                    r.h.subject_offset = ts;
+                                    r.h.hsp_len = x.len;
+                                    r.h.score = x.score;
+                                    r.j = tj;
+                                    r.pad = 0;
+                                    p.seg[seg_base + at] = r;
+                                } else {
+                                    atomicOr(p.overflow, 1u);
+                                }
+                            }
+                            seg_fill += (uint32_t)__popcll(rb);
+                        }
+                        ntasks = 0;
+                        wave_sync();
+                    }
                     if (tb) {
-                        const uint32_t nt = (uint32_t)__popcll(tb);
-                        if (ntasks + nt > kTaskCap) flush_tasks();
                         if (task) {
                             const uint32_t idx = ntasks + rank_below(tb);
                             t_slot[idx] = dslot;
@@ -390,22 +454,31 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                             t_s[idx] = (uint32_t)s;
                             t_sl[idx] = (uint32_t)sl;
                         }
-                        ntasks += nt;
+                        ntasks += (uint32_t)__popcll(tb);
                     }
                     if (write) diag[dslot] = ns;
                     wave_sync();
-                    if (__ballot(conflict)) flush_tasks();
                 }
                 head = (head + n) & qmask;
                 count -= n;
             };
-            auto drain_all = [&]() {
-                while (count > 0) drain_batch(min(count, 64u));
-            };
 
+            // Positions are consumed 64 at a time into the ring; the ring is
+            // drained whenever a full batch is ready, when it must be emptied
+            // (16-bit offset rebase, a cell that does not fit) and at the end.
             const int32_t last = slen - WORD;
-            for (int32_t pos = 0; pos <= last;) {
-                if (!resident && pos + 128 > whi && whi < send) {
+            int32_t pos = 0;
+            bool empty_ring = false;
+            for (;;) {
+                const bool at_end = pos > last;
+                if (count >= 64 || ((empty_ring || at_end) && count > 0) || (at_end && ntasks > 0)) {
+                    const uint32_t n = min(count, 64u);
+                    drain_batch(n, at_end && count <= 64);
+                    continue;
+                }
+                empty_ring = false;
+                if (at_end) break;
+                if (pos + 128 > whi && whi < send) {
                     const int32_t nhi = min(send, whi + (int32_t)(p.sbuf >> 1));
                     fill(whi, nhi);
                     whi = nhi;
@@ -414,8 +487,8 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 if (count == 0) {
                     qbase = pos;
                 } else if (pos + 64 - qbase >= 65536) {
-                    drain_all();
-                    qbase = pos;
+                    empty_ring = true;
+                    continue;
                 }
                 const int32_t s = pos + (int32_t)lane;
                 uint32_t cnt = 0, start = 0;
@@ -437,9 +510,10 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 if (nfit == 0) {
                     if (count == 0) {  // a cell larger than the ring: host sizing error
                         if (lane == 0) atomicOr(p.overflow, 2u);
-                        break;
+                        pos = last + 1;
+                        continue;
                     }
-                    drain_all();
+                    empty_ring = true;
                     continue;
                 }
                 if (fits && cnt) {
@@ -452,10 +526,7 @@ __global__ __launch_bounds__(1024) void k_scan(ScanParams p) {
                 hits += added;
                 pos += (int32_t)nfit;
                 wave_sync();
-                while (count >= 64) drain_batch(64);
             }
-            drain_all();
-            flush_tasks();
 
             // ---- per-subject outputs ----
             const uint32_t gslot = w.group * p.count + slot;

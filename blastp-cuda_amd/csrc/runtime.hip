@@ -112,6 +112,7 @@ struct bpx_ctx {
     int64_t lds_limit = 0;
     int64_t hbm_bytes = 0;
     uint32_t* work_counter = nullptr;  // one per launch class
+    uint32_t seg_hint = 0;             // per-wave record segment size a previous scan needed
     Pool pool;
     Pool pinned{true};
 };
@@ -340,7 +341,7 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_laun
     bpx_ctx* ctx = r->ctx;
     const bpx_launch L = resolve_launch(user);
     const bool auto_waves = !user || user->waves_per_group <= 0;
-    const int want_waves = auto_waves ? 8 : std::min(16, std::max(1, user->waves_per_group));
+    const int want_waves = auto_waves ? 8 : std::min(8, std::max(1, user->waves_per_group));
     uint32_t maxcell = 0;
     for (uint32_t c : set.max_cell) maxcell = std::max(maxcell, c);
     uint32_t qc = L.queue_capacity > 0 ? (uint32_t)L.queue_capacity : 256u;
@@ -546,6 +547,7 @@ int enqueue(bpx_result* r, const uint32_t* order) {
         sp.seg_cap = r->seg_cap;
         sp.seg_wave_base = (uint32_t)(seg_at / r->seg_cap);
         sp.overflow = overflow;
+        sp.max_fill = reinterpret_cast<uint32_t*>(r->d_meta + 2);
         sp.lds_table = c.plan.table;
         sp.lds_qres = c.plan.qres;
         sp.lds_qmap = c.plan.qmap;
@@ -619,7 +621,8 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
         }
     const uint64_t waves = std::max<uint64_t>(total_waves(r), 1);
     const uint64_t pairs = (uint64_t)set.nq * count;
-    const uint32_t seg_cap = (uint32_t)std::max<uint64_t>(64, (pairs / 8 + 65536) / waves + 1);
+    const uint32_t seg_cap =
+        (uint32_t)std::max<uint64_t>(std::max<uint64_t>(64, ctx->seg_hint), (pairs / 8 + 65536) / waves + 1);
     rc = alloc_buffers(r, seg_cap);
     if (rc) {
         bpx_result_destroy(r);
@@ -937,8 +940,11 @@ int bpx_result_wait(bpx_result* r) {
         const uint32_t ovf = (uint32_t)(r->h_meta[1] & 0xFFFFFFFFu);
         if (ovf & 2u) return fail(BPX_E_STATE, "internal: hit ring smaller than a lookup cell");
         if (!(ovf & 1u)) break;
-        // a wave's record segment filled: rescan with 4x the segment size
-        const uint32_t cap = r->seg_cap * 4;
+        // a wave's record segment filled: rescan with segments as large as
+        // the fullest wave needed (remembered for later scans)
+        const uint32_t need = (uint32_t)(r->h_meta[2] & 0xFFFFFFFFu);
+        const uint32_t cap = std::max<uint32_t>(need + need / 8 + 64, r->seg_cap * 2);
+        ctx->seg_hint = std::max(ctx->seg_hint, cap);
         const uint32_t* order = r->d_order ? r->d_order : r->db->d_order;
         result_release(r, true);
         int rc = alloc_buffers(r, cap);

@@ -226,7 +226,7 @@ def test_segment_overflow_rescan(ctx):
 
 
 @pytest.mark.parametrize("launch", [dict(waves_per_group=1), dict(waves_per_group=4, groups_per_cu=2),
-                                    dict(waves_per_group=16, subjects_per_item=8, subject_buffer=1024),
+                                    dict(waves_per_group=8, subjects_per_item=8, subject_buffer=1024),
                                     dict(queue_capacity=4096), dict(subject_buffer=1024),
                                     dict(subject_buffer=8192, waves_per_group=4)])
 def test_launch_shapes_do_not_change_results(ctx, launch):
@@ -245,8 +245,8 @@ def test_infeasible_launch_is_refused(ctx):
     db = ctx.load_db(res, offs[:-1], lens)
     dq = ctx.prepare_query(q)
     with pytest.raises(bpx.BpxError, match="LDS plan"):
-        bpx.scan(ctx, dq, db, launch=dict(waves_per_group=16, subject_buffer=65536))
-    rc, dev, lds = bpx.check_memory(ctx, dq, db, launch=dict(waves_per_group=16, subject_buffer=65536))
+        bpx.scan(ctx, dq, db, launch=dict(waves_per_group=8, subject_buffer=65536))
+    rc, dev, lds = bpx.check_memory(ctx, dq, db, launch=dict(waves_per_group=8, subject_buffer=65536))
     assert rc == -5 and lds > ctx.info()["lds_bytes"]
 
 
