@@ -86,8 +86,7 @@ struct TwoHit {
 // Subject residue p of the wave's staged subject.  sbuf holds positions
 // [wlo, whi) (all of a subject that fits); any other position -- behind the
 // window, or ahead of it when a right extension runs past the loaded part --
-// comes from HBM.  The LDS read is unconditional so the common case stays a
-// plain ds_read.
+// comes from HBM.
 __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
                                          int32_t wlo, int32_t whi, int32_t slen) {
     uint32_t v = sbuf[p & smask];
@@ -99,23 +98,32 @@ __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, co
 // loops of :435-481 and :503-549): best prefix of the second word, X-drop to
 // the left from one past it, to the right only if the left reached the
 // first hit.  c is the second hit's position in the group's concatenated
-// residues Q; the query occupies Q[qstart, qstart + qlen).
+// residues Q (qcap bytes); the query occupies Q[qstart, qstart + qlen).
 //
 // Both X-drop loops run in blocks of kBlock steps: the block's residues and
 // scores are loaded together, then the steps are applied in order in
-// registers with a `done` predicate, so each step still sees exactly the
-// sequential running score, maximum and stop test.
+// registers under a `done` predicate, so each step sees exactly the
+// sequential running score, maximum and stop test.  RESIDENT (the whole
+// subject is in sbuf): every load is an unconditional LDS read at a clamped
+// index, masked afterwards.
 constexpr int kBlock = 8;
 
-template <int WORD>
-__device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
-                                                 const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t wlo,
-                                                 int32_t whi, int32_t slen, int32_t s_left, int32_t s, int32_t c,
-                                                 int32_t xdrop) {
+template <bool RESIDENT>
+__device__ __forceinline__ uint32_t sread(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
+                                          int32_t wlo, int32_t whi, int32_t slen) {
+    if (RESIDENT) return sbuf[p & smask];
+    return sres(sbuf, smask, S, p, wlo, whi, slen);
+}
+
+template <int WORD, bool RESIDENT>
+__device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t* Q, uint32_t qcap, int32_t qstart,
+                                                 int32_t qlen, const uint8_t* sbuf, uint32_t smask,
+                                                 const uint8_t* S, int32_t wlo, int32_t whi, int32_t slen,
+                                                 int32_t s_left, int32_t s, int32_t c, int32_t xdrop) {
     int32_t sc = 0, best = 0, rd = 0;
 #pragma unroll
     for (int32_t i = 0; i < WORD; ++i) {
-        sc += M[Q[c + i] * kMatrixStride + sres(sbuf, smask, S, s + i, wlo, whi, slen)];
+        sc += M[Q[c + i] * kMatrixStride + sread<RESIDENT>(sbuf, smask, S, s + i, wlo, whi, slen)];
         if (sc > best) {
             best = sc;
             rd = i + 1;
@@ -129,29 +137,25 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     bool done = n < 0;
     for (int32_t k0 = 0; !done; k0 += kBlock) {
         int32_t m[kBlock];
+        uint32_t qb[kBlock], sb[kBlock];
 #pragma unroll
         for (int i = 0; i < kBlock; ++i) {
             const int32_t k = k0 + i;
-            const bool ok = k <= n;
-            const uint32_t qb = ok ? Q[cr - 1 - k] : 0u;
-            const uint32_t sb = ok ? sres(sbuf, smask, S, sr - 1 - k, wlo, whi, slen) : 0u;
-            m[i] = ok ? (int32_t)M[qb * kMatrixStride + sb] : 0;
+            qb[i] = Q[max(cr - 1 - k, 0)];
+            sb[i] = RESIDENT ? (uint32_t)sbuf[max(sr - 1 - k, 0) & smask]
+                             : (k <= n ? sres(sbuf, smask, S, sr - 1 - k, wlo, whi, slen) : 0u);
         }
 #pragma unroll
+        for (int i = 0; i < kBlock; ++i) m[i] = M[qb[i] * kMatrixStride + sb[i]];
+#pragma unroll
         for (int i = 0; i < kBlock; ++i) {
             const int32_t k = k0 + i;
-            if (!done) {
-                if (k > n) {
-                    done = true;
-                } else {
-                    score += m[i];
-                    if (score > mx) {
-                        mx = score;
-                        left_len = k + 1;
-                    }
-                    if (mx - score >= xdrop) done = true;
-                }
-            }
+            const bool ok = !done && k <= n;
+            score = ok ? score + m[i] : score;
+            const bool up = ok && score > mx;
+            mx = up ? score : mx;
+            left_len = up ? k + 1 : left_len;
+            done = done || k >= n || (ok && mx - score >= xdrop);
         }
     }
     TwoHit r;
@@ -165,33 +169,27 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
         bool rdone = n2 <= 0;
         for (int32_t k0 = 0; !rdone; k0 += kBlock) {
             int32_t m[kBlock];
+            uint32_t qb[kBlock], sb[kBlock];
 #pragma unroll
             for (int i = 0; i < kBlock; ++i) {
                 const int32_t k = k0 + i;
-                const bool ok = k < n2;
-                const uint32_t qb = ok ? Q[cr + k] : 0u;
-                const uint32_t sb = ok ? sres(sbuf, smask, S, sr + k, wlo, whi, slen) : 0u;
-                m[i] = ok ? (int32_t)M[qb * kMatrixStride + sb] : 0;
+                qb[i] = Q[min((uint32_t)(cr + k), qcap - 1)];
+                sb[i] = RESIDENT ? (uint32_t)sbuf[(sr + k) & smask]
+                                 : (k < n2 ? sres(sbuf, smask, S, sr + k, wlo, whi, slen) : 0u);
             }
 #pragma unroll
+            for (int i = 0; i < kBlock; ++i) m[i] = M[qb[i] * kMatrixStride + sb[i]];
+#pragma unroll
             for (int i = 0; i < kBlock; ++i) {
                 const int32_t k = k0 + i;
-                if (!rdone) {
-                    if (k >= n2) {
-                        rdone = true;
-                        kstop = n2;
-                    } else {
-                        rscore += m[i];
-                        if (rscore > rmax) {
-                            rmax = rscore;
-                            best_k = k;
-                        }
-                        if (rscore <= 0 || rmax - rscore >= xdrop) {
-                            rdone = true;
-                            kstop = k;
-                        }
-                    }
-                }
+                const bool ok = !rdone && k < n2;
+                rscore = ok ? rscore + m[i] : rscore;
+                const bool up = ok && rscore > rmax;
+                rmax = up ? rscore : rmax;
+                best_k = up ? k : best_k;
+                const bool stop = ok && (rscore <= 0 || rmax - rscore >= xdrop);
+                kstop = stop ? k : kstop;
+                rdone = rdone || stop || k + 1 >= n2;
             }
         }
         right_len = best_k + 1;
@@ -309,6 +307,7 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                 for (int32_t x = from + 16 * (int32_t)lane; x < to; x += 1024)
                     *reinterpret_cast<uint4*>(sbuf + (x & smask)) = *reinterpret_cast<const uint4*>(S + x);
             };
+            const bool resident = send <= (int32_t)p.sbuf;
             int32_t whi = min(send, (int32_t)p.sbuf);
             fill(0, whi);
             wave_sync();
@@ -325,13 +324,14 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
             // overflow, and (force) after the last batch of the subject.
             auto drain_batch = [&](uint32_t n, bool force) {
                 bool active = lane < n;
-                const uint32_t e = active ? ring[(head + lane) & qmask] : 0u;
+                const uint32_t e_raw = ring[(head + lane) & qmask];
+                const uint32_t e = active ? e_raw : 0u;
                 const int32_t c = (int32_t)(e >> 16);
                 const int32_t s = qbase + (int32_t)(e & 0xFFFFu);
                 uint32_t j = 0, dbase = 0, dmask = q0.dmask;
                 int32_t qstart = (int32_t)q0.qstart;
                 if (MULTI) {
-                    j = active ? qmap[c] : 0u;
+                    j = qmap[c];
                     const QueryInfo qi = qinfo[j];
                     dbase = qi.dbase;
                     dmask = qi.dmask;
@@ -407,8 +407,14 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                             ts = (int32_t)t_s[lane];
                             const int32_t tsl = (int32_t)t_sl[lane];
                             const QueryInfo qi = MULTI ? qinfo[tj] : q0;
-                            x = extend_two_hit<WORD>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, S,
-                                                     whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc, qi.xdrop);
+                            if (resident)
+                                x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
+                                                               sbuf, smask, S, 0, whi, slen, tsl, ts, tc, qi.xdrop);
+                            else
+                                x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart,
+                                                                (int32_t)qi.qlen, sbuf, smask, S,
+                                                                whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc,
+                                                                qi.xdrop);
                             rec = x.score >= qi.cutoff;
                             diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
                                                          : (uint32_t)(ts + doff);
@@ -491,19 +497,17 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                     continue;
                 }
                 const int32_t s = pos + (int32_t)lane;
-                uint32_t cnt = 0, start = 0;
-                if (s <= last) {
-                    uint32_t wd = sbuf[s & smask];
+                uint32_t wd = sbuf[s & smask];
 #pragma unroll
-                    for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
-                    const uint2 gv = pvr[wd >> 5];
-                    const uint32_t bit = 1u << (wd & 31u);
-                    if (gv.x & bit) {
-                        const uint32_t r = gv.y + (uint32_t)__popc(gv.x & (bit - 1u));
-                        start = starts[r];
-                        cnt = starts[r + 1] - start;
-                    }
-                }
+                for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
+                const uint2 gv = pvr[wd >> 5];
+                const uint32_t bit = 1u << (wd & 31u);
+                const bool present = (gv.x & bit) && s <= last;
+                // rank of the word among present words; an absent word reads
+                // the neighbouring (valid) start entries and is masked out
+                const uint32_t r = min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
+                const uint32_t start = starts[r];
+                const uint32_t cnt = present ? starts[r + 1] - start : 0u;
                 const uint32_t incl = wave_inclusive_sum(cnt);
                 const bool fits = incl <= qcap - count;
                 const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));

@@ -296,6 +296,10 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
         if (keep_tables) keep_tables->push_back(std::move(t));
     }
     set.table_bytes = (uint64_t)tables.size() * 4;
+    // zero words after the last table: the scan reads start entries at a
+    // clamped rank without a branch and may touch one word past a group
+    tables.insert(tables.end(), 16, 0u);
+    qres.insert(qres.end(), 16, 0u);
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t ng = std::max<size_t>(set.groups.size(), 1), nqi = std::max<size_t>(set.qinfo.size(), 1);
     if (hipMalloc(&set.d_groups, ng * sizeof(bpx::GroupInfo)) != hipSuccess ||
