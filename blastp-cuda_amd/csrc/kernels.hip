@@ -203,6 +203,111 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     return r;
 }
 
+// Inclusive prefix sum / maximum inside each 16-lane row (DPP row_shr
+// 1, 2, 4, 8; lanes shifted in from outside the row contribute the identity).
+__device__ __forceinline__ int32_t row_incl_sum(int32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    return v;
+}
+__device__ __forceinline__ int32_t row_incl_max(int32_t v) {
+    constexpr int32_t lo = (int32_t)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xf, false));
+    return v;
+}
+
+// One X-drop pass of a row: 16 consecutive steps per iteration.  Step k of
+// the row has score m_k; the running score is carry + prefix sum, the
+// running maximum the prefix maximum, and the first lane whose stop test
+// holds (or the first step past the end) ends the pass -- the sequential
+// loop of gpu_blastp_kernel.cu:451-474 (left) / :523-541 (right), evaluated
+// 16 steps at a time.  RIGHT adds the stop on a non-positive running score.
+// Returns (via refs) the maximum, the step of its first occurrence (-1 if it
+// never rose above the start value) and the stop step.
+template <bool RIGHT>
+__device__ __forceinline__ void row_xdrop(const int8_t* M, const uint8_t* Q, uint32_t qcap, const uint8_t* sbuf,
+                                          uint32_t smask, int32_t cq, int32_t cs, int32_t nsteps, int32_t start,
+                                          int32_t xdrop, uint32_t lane, int32_t& out_max, int32_t& out_best,
+                                          int32_t& out_stop) {
+    constexpr int32_t kNeg = -(1 << 22);
+    const uint32_t rl = lane & 15u, rbase = lane & ~15u;
+    int32_t carry = start, mx = start, best = -1, stop_at = nsteps;
+    bool done = nsteps <= 0;
+    for (int32_t k0 = 0; __ballot(!done) != 0; k0 += 16) {
+        const int32_t k = k0 + (int32_t)rl;
+        const bool ok = !done && k < nsteps;
+        // left steps walk down from (cq, cs), right steps walk up from it
+        const int32_t qi = RIGHT ? cq + k : cq - k;
+        const int32_t si = RIGHT ? cs + k : cs - k;
+        const uint32_t qb = Q[min((uint32_t)max(qi, 0), qcap - 1)];
+        const uint32_t sb = sbuf[(uint32_t)max(si, 0) & smask];
+        const int32_t m = ok ? (int32_t)M[qb * kMatrixStride + sb] : kNeg;
+        const int32_t run = carry + row_incl_sum(m);
+        const int32_t pm = max(mx, row_incl_max(run));
+        const bool stop = ok ? ((RIGHT && run <= 0) || pm - run >= xdrop) : true;
+        const uint32_t smaskr = (uint32_t)(__ballot(stop) >> rbase) & 0xFFFFu;
+        const uint32_t kb = smaskr ? (uint32_t)__ffs(smaskr) - 1u : 15u;
+        const int32_t pm_kb = __shfl(pm, (int)(rbase + kb), 64);
+        const int32_t run_last = __shfl(run, (int)(rbase + 15u), 64);
+        const uint32_t eq = (uint32_t)(__ballot(run == pm_kb && rl <= kb) >> rbase) & 0xFFFFu;
+        if (!done) {
+            if (pm_kb > mx) {
+                mx = pm_kb;
+                best = k0 + (int32_t)__ffs(eq) - 1;
+            }
+            carry = run_last;
+            if (smaskr) {
+                done = true;
+                stop_at = k0 + (int32_t)kb;
+            }
+        }
+    }
+    out_max = mx;
+    out_best = best;
+    out_stop = stop_at;
+}
+
+// s_BlastAaExtendTwoHit for one task per 16-lane row (all 64 lanes call
+// it; a row with valid == false does no work).  Same result as
+// extend_two_hit for a subject resident in sbuf.
+template <int WORD>
+__device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uint8_t* Q, uint32_t qcap,
+                                                      const uint8_t* sbuf, uint32_t smask, int32_t slen, bool valid,
+                                                      int32_t qstart, int32_t qlen, int32_t s_left, int32_t s,
+                                                      int32_t c, int32_t xdrop, uint32_t lane) {
+    int32_t sc = 0, best = 0, rd = 0;
+#pragma unroll
+    for (int32_t i = 0; i < WORD; ++i) {
+        sc += M[Q[min((uint32_t)(c + i), qcap - 1)] * kMatrixStride + sbuf[(uint32_t)(s + i) & smask]];
+        if (sc > best) {
+            best = sc;
+            rd = i + 1;
+        }
+    }
+    const int32_t cr = c + rd, sr = s + rd, qr = cr - qstart;
+    // left: steps k = 0..n at (cr-1-k, sr-1-k)
+    const int32_t nl = valid ? min(sr - 1, qr - 1) + 1 : 0;
+    int32_t lmax, lbest, lstop;
+    row_xdrop<false>(M, Q, qcap, sbuf, smask, cr - 1, sr - 1, nl, 0, xdrop, lane, lmax, lbest, lstop);
+    const int32_t left_len = lbest + 1;
+    TwoHit r;
+    r.right_extend = valid && left_len >= sr - s_left;
+    const int32_t nr = r.right_extend ? min(slen - sr, qlen - qr) : 0;
+    int32_t rmax, rbest, rstop;
+    row_xdrop<true>(M, Q, qcap, sbuf, smask, cr, sr, nr, lmax, xdrop, lane, rmax, rbest, rstop);
+    r.s_last = r.right_extend ? sr + rstop : sr;
+    r.hsp_q = qr - left_len;
+    r.hsp_s = sr - left_len;
+    r.len = left_len + (r.right_extend ? rbest + 1 : 0);
+    r.score = r.right_extend ? rmax : lmax;
+    return r;
+}
+
 // Per-query counters of a wave (MULTI groups), 32 bytes each.
 struct QCount {
     uint32_t hits, ext, saved, pad;
@@ -392,62 +497,76 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                     }
                     if (need_flush && ntasks > 0) {
                         // ---- run the queued extensions ----
+                        // A few tasks run on 16-lane rows (4 per pass);
+                        // many run one per lane.
                         const uint32_t nt = ntasks;
                         wave_sync();
-                        const bool mine = lane < nt;
-                        bool rec = false;
-                        TwoHit x{};
-                        uint32_t tj = 0;
-                        int32_t ts = 0, tc = 0;
-                        if (mine) {
-                            const uint32_t tslot = t_slot[lane];
-                            const uint32_t cj = t_cj[lane];
-                            tc = (int32_t)(cj >> 16);
-                            tj = cj & 0xFFFFu;
-                            ts = (int32_t)t_s[lane];
-                            const int32_t tsl = (int32_t)t_sl[lane];
+                        const bool coop = resident && nt <= 16;
+                        const uint32_t npass = coop ? (nt + 3) / 4 : 1;
+                        for (uint32_t pass = 0; pass < npass; ++pass) {
+                            const uint32_t t = coop ? pass * 4 + (lane >> 4) : lane;
+                            const bool tvalid = t < nt;
+                            const bool mine = coop ? (tvalid && (lane & 15u) == 0) : tvalid;
+                            const uint32_t ti = min(t, kTaskCap - 1);
+                            const uint32_t tslot = t_slot[ti];
+                            const uint32_t cj = t_cj[ti];
+                            int32_t tc = (int32_t)(cj >> 16);
+                            const uint32_t tj = cj & 0xFFFFu;
+                            const int32_t ts = (int32_t)t_s[ti];
+                            const int32_t tsl = (int32_t)t_sl[ti];
                             const QueryInfo qi = MULTI ? qinfo[tj] : q0;
-                            if (resident)
-                                x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
-                                                               sbuf, smask, S, 0, whi, slen, tsl, ts, tc, qi.xdrop);
-                            else
-                                x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart,
-                                                                (int32_t)qi.qlen, sbuf, smask, S,
-                                                                whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc,
-                                                                qi.xdrop);
-                            rec = x.score >= qi.cutoff;
-                            diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
-                                                         : (uint32_t)(ts + doff);
-                            if (MULTI) {
-                                atomicAdd(&qc[tj].ext, 1u);
-                                if (rec) atomicAdd(&qc[tj].saved, 1u);
+                            TwoHit x{};
+                            if (coop) {
+                                x = coop_extend_two_hit<WORD>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid,
+                                                              (int32_t)qi.qstart, (int32_t)qi.qlen, tsl, ts, tc,
+                                                              qi.xdrop, lane);
+                            } else if (mine) {
+                                if (resident)
+                                    x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart,
+                                                                   (int32_t)qi.qlen, sbuf, smask, S, 0, whi, slen,
+                                                                   tsl, ts, tc, qi.xdrop);
+                                else
+                                    x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart,
+                                                                    (int32_t)qi.qlen, sbuf, smask, S,
+                                                                    whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc,
+                                                                    qi.xdrop);
                             }
-                            tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
-                        }
-                        const unsigned long long rb = __ballot(rec);
-                        if (!MULTI) {
-                            extended += nt;
-                            saved += (uint32_t)__popcll(rb);
-                        }
-                        if (rb) {
-                            const uint32_t at = seg_fill + rank_below(rb);
-                            if (rec) {
-                                if (at < p.seg_cap) {
-                                    SegRecord r;
-                                    r.h.hsp_q = x.hsp_q;
-                                    r.h.hsp_s = x.hsp_s;
-                                    r.h.query_offset = tc;
-                                    r.h.subject_offset = ts;
-                                    r.h.hsp_len = x.len;
-                                    r.h.score = x.score;
-                                    r.j = tj;
-                                    r.pad = 0;
-                                    p.seg[seg_base + at] = r;
-                                } else {
-                                    atomicOr(p.overflow, 1u);
+                            bool rec = false;
+                            if (mine) {
+                                rec = x.score >= qi.cutoff;
+                                diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
+                                                             : (uint32_t)(ts + doff);
+                                if (MULTI) {
+                                    atomicAdd(&qc[tj].ext, 1u);
+                                    if (rec) atomicAdd(&qc[tj].saved, 1u);
                                 }
+                                tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
                             }
-                            seg_fill += (uint32_t)__popcll(rb);
+                            const unsigned long long rb = __ballot(rec);
+                            if (!MULTI) {
+                                extended += (uint32_t)__popcll(__ballot(mine));
+                                saved += (uint32_t)__popcll(rb);
+                            }
+                            if (rb) {
+                                const uint32_t at = seg_fill + rank_below(rb);
+                                if (rec) {
+                                    if (at < p.seg_cap) {
+                                        SegRecord r;
+                                        r.h.hsp_q = x.hsp_q;
+                                        r.h.hsp_s = x.hsp_s;
+                                        r.h.query_offset = tc;
+                                        r.h.subject_offset = ts;
+                                        r.h.hsp_len = x.len;
+                                        r.h.score = x.score;
+                                        r.j = tj;
+                                        r.pad = 0;
+                                        p.seg[seg_base + at] = r;
+                                    } else {
+                                        atomicOr(p.overflow, 1u);
+                                    }
+                                }
+                                seg_fill += (uint32_t)__popcll(rb);
+                            }
                         }
                         ntasks = 0;
                         wave_sync();
