@@ -21,10 +21,12 @@
 //     atomic-min on the lane number (gpu_blastp_kernel.cu:194-301 applied
 //     exactly, in parallel across slots);
 //   * a hit that completes a pair does not extend inline: its slot is marked
-//     pending and the extension is queued.  Queued extensions run 64 at a
-//     time (s_BlastAaExtendTwoHit/Left/Right semantics, gpu_blastp_kernel.cu:
-//     341-549); a later hit on a pending slot first flushes the queue, so
-//     every slot still sees its hits strictly in scan order;
+//     pending and the extension is queued.  A later hit on a pending slot is
+//     deferred, in order, and applied again after the queue is flushed, so
+//     every slot still sees its hits strictly in scan order.  Queued
+//     extensions (s_BlastAaExtendTwoHit/Left/Right semantics,
+//     gpu_blastp_kernel.cu:341-549) run when a queue fills or the subject
+//     ends: one per lane, or a few on 16-lane rows;
 //   * records go to the wave's private output segment; per-(query, subject)
 //     counts, optional counters and per-query ungapped statistics are written
 //     when the subject is done.
@@ -343,7 +345,8 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
     uint32_t* t_cj = t_slot + kTaskCap;
     uint32_t* t_s = t_cj + kTaskCap;
     uint32_t* t_sl = t_s + kTaskCap;
-    QCount* qc = reinterpret_cast<QCount*>(t_sl + kTaskCap);
+    uint32_t* dq = t_sl + kTaskCap;
+    QCount* qc = reinterpret_cast<QCount*>(dq + kDefCap);
     for (uint32_t i = lane; i < p.diag_cap; i += 64) diag[i] = 0;
     for (uint32_t i = lane; i < kClaimSlots; i += 64) claim[i] = 0xFFFFFFFFu;
     if (MULTI)
@@ -419,21 +422,26 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
 
             uint32_t hits = 0, extended = 0, saved = 0;   // single-query counters
             const uint32_t rec_start = seg_fill;
-            uint32_t head = 0, count = 0, ntasks = 0;     // wave-uniform
+            uint32_t head = 0, count = 0;                 // hit ring (wave-uniform)
+            uint32_t ntasks = 0, ndef = 0;                // queued extensions, deferred hits
+            uint32_t rep = 0, rep_total = 0;              // deferred hits being re-applied
             int32_t qbase = 0;
 
-            // Applies the oldest n (<= 64) ring entries to the diagonal
-            // table.  Completed pairs are queued; the queue is flushed -- the
-            // extensions run one per lane and their outcome is written back --
-            // when a hit lands on a pending slot, when the queue would
-            // overflow, and (force) after the last batch of the subject.
-            auto drain_batch = [&](uint32_t n, bool force) {
+            // Applies n (<= 64) hits to the diagonal table; lane i holds the
+            // i-th in scan order.  Hits on one slot are applied in lane order
+            // (the lowest lane wins a hashed claim each round).  A hit that
+            // completes a pair marks its slot pending and queues the
+            // extension; a hit that finds its slot pending is deferred --
+            // appended, in order, to the deferred queue, which is applied
+            // again right after the next flush.  So every slot still sees its
+            // hits strictly in scan order, and a flush is needed only when a
+            // queue is full or the subject ends.  The caller guarantees
+            // ntasks + n <= kTaskCap and ndef + n <= kDefCap.
+            auto apply = [&](uint32_t e, uint32_t n, bool fresh) {
                 bool active = lane < n;
-                const uint32_t e_raw = ring[(head + lane) & qmask];
-                const uint32_t e = active ? e_raw : 0u;
                 const int32_t c = (int32_t)(e >> 16);
                 const int32_t s = qbase + (int32_t)(e & 0xFFFFu);
-                uint32_t j = 0, dbase = 0, dmask = q0.dmask;
+                uint32_t j = 0, dbase = q0.dbase, dmask = q0.dmask;
                 int32_t qstart = (int32_t)q0.qstart;
                 if (MULTI) {
                     j = qmap[c];
@@ -441,136 +449,48 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                     dbase = qi.dbase;
                     dmask = qi.dmask;
                     qstart = (int32_t)qi.qstart;
-                    if (active) atomicAdd(&qc[j].hits, 1u);
-                } else {
-                    dbase = q0.dbase;
+                    if (active && fresh) atomicAdd(&qc[j].hits, 1u);
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
                 const uint32_t h = dslot & (kClaimSlots - 1);
-                bool force_left = force;
-                for (;;) {
-                    const bool any = __ballot(active) != 0;
-                    if (!any && !(force_left && ntasks > 0)) break;
-                    bool conflict = false, task = false, write = false;
+                bool deferred = false;
+                while (__ballot(active) != 0) {
+                    if (active) atomicMin(&claim[h], lane);
+                    wave_sync();
+                    const bool win = active && claim[h] == lane;
+                    wave_sync();
+                    bool task = false, write = false;
                     uint32_t ns = 0;
                     int32_t sl = 0;
-                    bool need_flush;
-                    unsigned long long tb = 0;
-                    if (any) {
-                        if (active) atomicMin(&claim[h], lane);
-                        wave_sync();
-                        const bool win = active && claim[h] == lane;
-                        wave_sync();
-                        if (win) {
-                            claim[h] = 0xFFFFFFFFu;
-                            const uint32_t st = diag[dslot];
-                            if (st & kPend) {
-                                conflict = true;
+                    if (win) {
+                        claim[h] = 0xFFFFFFFFu;
+                        active = false;
+                        const uint32_t st = diag[dslot];
+                        if (st & kPend) {
+                            deferred = true;
+                        } else {
+                            const int32_t v = (int32_t)(st & kVal);
+                            if (st & kFlag) {
+                                if (s + doff >= v) {
+                                    ns = (uint32_t)(s + doff);
+                                    write = true;
+                                }
                             } else {
-                                active = false;
-                                const int32_t v = (int32_t)(st & kVal);
-                                if (st & kFlag) {
-                                    if (s + doff >= v) {
-                                        ns = (uint32_t)(s + doff);
-                                        write = true;
-                                    }
-                                } else {
-                                    const int32_t lh = v - doff;
-                                    const int32_t diff = s - lh;
-                                    if (diff >= window) {
-                                        ns = (uint32_t)(s + doff);
-                                        write = true;
-                                    } else if (diff >= WORD) {
-                                        task = true;
-                                        sl = lh + WORD;
-                                        ns = kPend;
-                                        write = true;
-                                    }
+                                const int32_t lh = v - doff;
+                                const int32_t diff = s - lh;
+                                if (diff >= window) {
+                                    ns = (uint32_t)(s + doff);
+                                    write = true;
+                                } else if (diff >= WORD) {
+                                    task = true;
+                                    sl = lh + WORD;
+                                    ns = kPend;
+                                    write = true;
                                 }
                             }
                         }
-                        tb = __ballot(task);
-                        need_flush = (ntasks + (uint32_t)__popcll(tb) > kTaskCap) || __ballot(conflict) != 0;
-                    } else {
-                        need_flush = true;
-                        force_left = false;
                     }
-                    if (need_flush && ntasks > 0) {
-                        // ---- run the queued extensions ----
-                        // A few tasks run on 16-lane rows (4 per pass);
-                        // many run one per lane.
-                        const uint32_t nt = ntasks;
-                        wave_sync();
-                        const bool coop = resident && nt <= 16;
-                        const uint32_t npass = coop ? (nt + 3) / 4 : 1;
-                        for (uint32_t pass = 0; pass < npass; ++pass) {
-                            const uint32_t t = coop ? pass * 4 + (lane >> 4) : lane;
-                            const bool tvalid = t < nt;
-                            const bool mine = coop ? (tvalid && (lane & 15u) == 0) : tvalid;
-                            const uint32_t ti = min(t, kTaskCap - 1);
-                            const uint32_t tslot = t_slot[ti];
-                            const uint32_t cj = t_cj[ti];
-                            int32_t tc = (int32_t)(cj >> 16);
-                            const uint32_t tj = cj & 0xFFFFu;
-                            const int32_t ts = (int32_t)t_s[ti];
-                            const int32_t tsl = (int32_t)t_sl[ti];
-                            const QueryInfo qi = MULTI ? qinfo[tj] : q0;
-                            TwoHit x{};
-                            if (coop) {
-                                x = coop_extend_two_hit<WORD>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid,
-                                                              (int32_t)qi.qstart, (int32_t)qi.qlen, tsl, ts, tc,
-                                                              qi.xdrop, lane);
-                            } else if (mine) {
-                                if (resident)
-                                    x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart,
-                                                                   (int32_t)qi.qlen, sbuf, smask, S, 0, whi, slen,
-                                                                   tsl, ts, tc, qi.xdrop);
-                                else
-                                    x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart,
-                                                                    (int32_t)qi.qlen, sbuf, smask, S,
-                                                                    whi - (int32_t)p.sbuf, whi, slen, tsl, ts, tc,
-                                                                    qi.xdrop);
-                            }
-                            bool rec = false;
-                            if (mine) {
-                                rec = x.score >= qi.cutoff;
-                                diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
-                                                             : (uint32_t)(ts + doff);
-                                if (MULTI) {
-                                    atomicAdd(&qc[tj].ext, 1u);
-                                    if (rec) atomicAdd(&qc[tj].saved, 1u);
-                                }
-                                tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
-                            }
-                            const unsigned long long rb = __ballot(rec);
-                            if (!MULTI) {
-                                extended += (uint32_t)__popcll(__ballot(mine));
-                                saved += (uint32_t)__popcll(rb);
-                            }
-                            if (rb) {
-                                const uint32_t at = seg_fill + rank_below(rb);
-                                if (rec) {
-                                    if (at < p.seg_cap) {
-                                        SegRecord r;
-                                        r.h.hsp_q = x.hsp_q;
-                                        r.h.hsp_s = x.hsp_s;
-                                        r.h.query_offset = tc;
-                                        r.h.subject_offset = ts;
-                                        r.h.hsp_len = x.len;
-                                        r.h.score = x.score;
-                                        r.j = tj;
-                                        r.pad = 0;
-                                        p.seg[seg_base + at] = r;
-                                    } else {
-                                        atomicOr(p.overflow, 1u);
-                                    }
-                                }
-                                seg_fill += (uint32_t)__popcll(rb);
-                            }
-                        }
-                        ntasks = 0;
-                        wave_sync();
-                    }
+                    const unsigned long long tb = __ballot(task);
                     if (tb) {
                         if (task) {
                             const uint32_t idx = ntasks + rank_below(tb);
@@ -584,71 +504,175 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                     if (write) diag[dslot] = ns;
                     wave_sync();
                 }
-                head = (head + n) & qmask;
-                count -= n;
+                const unsigned long long db = __ballot(deferred);
+                if (deferred) dq[ndef + rank_below(db)] = e;
+                ndef += (uint32_t)__popcll(db);
+                wave_sync();
             };
 
-            // Positions are consumed 64 at a time into the ring; the ring is
-            // drained whenever a full batch is ready, when it must be emptied
-            // (16-bit offset rebase, a cell that does not fit) and at the end.
+            // Runs the queued extensions and writes their outcome to the
+            // diagonal table.  A few tasks run on 16-lane rows (4 per pass);
+            // more run one per lane.
+            auto flush = [&]() {
+                const uint32_t nt = ntasks;
+                wave_sync();
+                const bool coop = resident && nt <= 16;
+                const uint32_t npass = coop ? (nt + 3) / 4 : (nt + 63) / 64;
+                for (uint32_t pass = 0; pass < npass; ++pass) {
+                    const uint32_t t = coop ? pass * 4 + (lane >> 4) : pass * 64 + lane;
+                    const bool tvalid = t < nt;
+                    const bool mine = coop ? (tvalid && (lane & 15u) == 0) : tvalid;
+                    const uint32_t ti = min(t, kTaskCap - 1);
+                    const uint32_t tslot = t_slot[ti];
+                    const uint32_t cj = t_cj[ti];
+                    int32_t tc = (int32_t)(cj >> 16);
+                    const uint32_t tj = cj & 0xFFFFu;
+                    const int32_t ts = (int32_t)t_s[ti];
+                    const int32_t tsl = (int32_t)t_sl[ti];
+                    const QueryInfo qi = MULTI ? qinfo[tj] : q0;
+                    TwoHit x{};
+                    if (coop) {
+                        x = coop_extend_two_hit<WORD>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid, (int32_t)qi.qstart,
+                                                      (int32_t)qi.qlen, tsl, ts, tc, qi.xdrop, lane);
+                    } else if (mine) {
+                        if (resident)
+                            x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
+                                                           sbuf, smask, S, 0, whi, slen, tsl, ts, tc, qi.xdrop);
+                        else
+                            x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
+                                                            sbuf, smask, S, whi - (int32_t)p.sbuf, whi, slen, tsl, ts,
+                                                            tc, qi.xdrop);
+                    }
+                    bool rec = false;
+                    if (mine) {
+                        rec = x.score >= qi.cutoff;
+                        diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
+                                                     : (uint32_t)(ts + doff);
+                        if (MULTI) {
+                            atomicAdd(&qc[tj].ext, 1u);
+                            if (rec) atomicAdd(&qc[tj].saved, 1u);
+                        }
+                        tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
+                    }
+                    const unsigned long long rb = __ballot(rec);
+                    if (!MULTI) {
+                        extended += (uint32_t)__popcll(__ballot(mine));
+                        saved += (uint32_t)__popcll(rb);
+                    }
+                    if (rb) {
+                        const uint32_t at = seg_fill + rank_below(rb);
+                        if (rec) {
+                            if (at < p.seg_cap) {
+                                SegRecord r;
+                                r.h.hsp_q = x.hsp_q;
+                                r.h.hsp_s = x.hsp_s;
+                                r.h.query_offset = tc;
+                                r.h.subject_offset = ts;
+                                r.h.hsp_len = x.len;
+                                r.h.score = x.score;
+                                r.j = tj;
+                                r.pad = 0;
+                                p.seg[seg_base + at] = r;
+                            } else {
+                                atomicOr(p.overflow, 1u);
+                            }
+                        }
+                        seg_fill += (uint32_t)__popcll(rb);
+                    }
+                }
+                ntasks = 0;
+                wave_sync();
+            };
+
+            // Positions are consumed 64 at a time into the ring.  Each pass
+            // of the loop does one of: re-apply up to 64 deferred hits (right
+            // after a flush, before anything else); flush (a queue cannot
+            // take the next batch, or everything must be applied: subject
+            // end, 16-bit offset rebase); apply the next ring batch (a full
+            // batch is ready, or the ring must be emptied); fill the ring
+            // from the next 64 positions.
             const int32_t last = slen - WORD;
             int32_t pos = 0;
-            bool empty_ring = false;
+            bool empty_ring = false, full_drain = false;
             for (;;) {
-                const bool at_end = pos > last;
-                if (count >= 64 || ((empty_ring || at_end) && count > 0) || (at_end && ntasks > 0)) {
-                    const uint32_t n = min(count, 64u);
-                    drain_batch(n, at_end && count <= 64);
-                    continue;
-                }
-                empty_ring = false;
-                if (at_end) break;
-                if (pos + 128 > whi && whi < send) {
-                    const int32_t nhi = min(send, whi + (int32_t)(p.sbuf >> 1));
-                    fill(whi, nhi);
-                    whi = nhi;
-                    wave_sync();
-                }
-                if (count == 0) {
-                    qbase = pos;
-                } else if (pos + 64 - qbase >= 65536) {
-                    empty_ring = true;
-                    continue;
-                }
-                const int32_t s = pos + (int32_t)lane;
-                uint32_t wd = sbuf[s & smask];
-#pragma unroll
-                for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
-                const uint2 gv = pvr[wd >> 5];
-                const uint32_t bit = 1u << (wd & 31u);
-                const bool present = (gv.x & bit) && s <= last;
-                // rank of the word among present words; an absent word reads
-                // the neighbouring (valid) start entries and is masked out
-                const uint32_t r = min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
-                const uint32_t start = starts[r];
-                const uint32_t cnt = present ? starts[r + 1] - start : 0u;
-                const uint32_t incl = wave_inclusive_sum(cnt);
-                const bool fits = incl <= qcap - count;
-                const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
-                if (nfit == 0) {
-                    if (count == 0) {  // a cell larger than the ring: host sizing error
-                        if (lane == 0) atomicOr(p.overflow, 2u);
-                        pos = last + 1;
+                uint32_t n, e;
+                bool fresh = false;
+                if (rep < rep_total) {
+                    n = min(64u, rep_total - rep);
+                    e = lane < n ? dq[rep + lane] : 0u;
+                    rep += n;
+                } else {
+                    const bool at_end = pos > last;
+                    const bool drain_all = at_end || full_drain;
+                    n = min(count, 64u);
+                    const bool batch = count >= 64 || ((empty_ring || drain_all) && count > 0);
+                    if (batch ? (ntasks + n > kTaskCap || ndef + n > kDefCap) : (drain_all && ntasks > 0)) {
+                        flush();
+                        rep = 0;
+                        rep_total = ndef;
+                        ndef = 0;
                         continue;
                     }
-                    empty_ring = true;
-                    continue;
+                    if (!batch) {
+                        empty_ring = full_drain = false;
+                        if (at_end) break;
+                        if (pos + 128 > whi && whi < send) {
+                            const int32_t nhi = min(send, whi + (int32_t)(p.sbuf >> 1));
+                            fill(whi, nhi);
+                            whi = nhi;
+                            wave_sync();
+                        }
+                        if (count == 0 && ndef == 0) {
+                            qbase = pos;
+                        } else if (pos + 64 - qbase >= 65536) {
+                            empty_ring = full_drain = true;
+                            continue;
+                        }
+                        const int32_t s = pos + (int32_t)lane;
+                        uint32_t wd = sbuf[s & smask];
+#pragma unroll
+                        for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
+                        const uint2 gv = pvr[wd >> 5];
+                        const uint32_t bit = 1u << (wd & 31u);
+                        const bool present = (gv.x & bit) && s <= last;
+                        // rank of the word among present words; an absent word
+                        // reads the neighbouring (valid) start entries and is
+                        // masked out
+                        const uint32_t r =
+                            min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
+                        const uint32_t start = starts[r];
+                        const uint32_t cnt = present ? starts[r + 1] - start : 0u;
+                        const uint32_t incl = wave_inclusive_sum(cnt);
+                        const bool fits = incl <= qcap - count;
+                        const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
+                        if (nfit == 0) {
+                            if (count == 0) {  // a cell larger than the ring: host sizing error
+                                if (lane == 0) atomicOr(p.overflow, 2u);
+                                pos = last + 1;
+                                continue;
+                            }
+                            empty_ring = true;
+                            continue;
+                        }
+                        if (fits && cnt) {
+                            const uint32_t at = head + count + (incl - cnt);
+                            const uint32_t sv = (uint32_t)(s - qbase);
+                            for (uint32_t t = 0; t < cnt; ++t)
+                                ring[(at + t) & qmask] = ((uint32_t)entries[start + t] << 16) | sv;
+                        }
+                        const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
+                        count += added;
+                        hits += added;
+                        pos += (int32_t)nfit;
+                        wave_sync();
+                        continue;
+                    }
+                    e = lane < n ? ring[(head + lane) & qmask] : 0u;
+                    head = (head + n) & qmask;
+                    count -= n;
+                    fresh = true;
                 }
-                if (fits && cnt) {
-                    const uint32_t at = head + count + (incl - cnt);
-                    const uint32_t sv = (uint32_t)(s - qbase);
-                    for (uint32_t t = 0; t < cnt; ++t) ring[(at + t) & qmask] = ((uint32_t)entries[start + t] << 16) | sv;
-                }
-                const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
-                count += added;
-                hits += added;
-                pos += (int32_t)nfit;
-                wave_sync();
+                apply(e, n, fresh);
             }
 
             // ---- per-subject outputs ----
@@ -854,7 +878,7 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     LdsPlan pl{};
     pl.max_nq = max_nq < 1 ? 1 : max_nq;
     pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
-    pl.wave_stride = up16(sbuf) + up16(diag_cap * 4) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 +
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * 4) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 + kDefCap * 4 +
                      (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
     uint32_t at = kCtlBytes;
     pl.table = at;

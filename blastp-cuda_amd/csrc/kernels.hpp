@@ -99,7 +99,8 @@ struct LdsPlan {
 };
 
 constexpr uint32_t kClaimSlots = 256;  // hashed claim table entries per wave
-constexpr uint32_t kTaskCap = 64;      // deferred extensions per wave
+constexpr uint32_t kTaskCap = 128;     // queued extensions per wave
+constexpr uint32_t kDefCap = 128;      // deferred hits per wave (hits on pending slots)
 
 LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
                  uint32_t qcap, uint32_t sbuf, int waves, bool table_in_lds);
