@@ -72,11 +72,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="first run a plain elementwise copy (torch.neg over int32) reading and writing BYTES, "
+                         "3 times: the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder shards by "
               "subject across GPUs, but this build measures the single-GPU path.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe // 4
+        a = torch.arange(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log(f"copy probe: {3} x torch.neg over {n} int32 ({4 * n} B read, {4 * n} B written per launch)")
+        del a, b
 
     import bpx
     log("generating config-2 workload ...")

@@ -454,42 +454,32 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
                 const uint32_t h = dslot & (kClaimSlots - 1);
                 bool deferred = false;
+                const int32_t sd = s + doff;
                 while (__ballot(active) != 0) {
                     if (active) atomicMin(&claim[h], lane);
                     wave_sync();
-                    const bool win = active && claim[h] == lane;
+                    // slot state and claim are read by every lane, together;
+                    // the state machine below is evaluated without branches
+                    const uint32_t cl = claim[h];
+                    const uint32_t st = diag[dslot];
                     wave_sync();
-                    bool task = false, write = false;
-                    uint32_t ns = 0;
-                    int32_t sl = 0;
-                    if (win) {
-                        claim[h] = 0xFFFFFFFFu;
-                        active = false;
-                        const uint32_t st = diag[dslot];
-                        if (st & kPend) {
-                            deferred = true;
-                        } else {
-                            const int32_t v = (int32_t)(st & kVal);
-                            if (st & kFlag) {
-                                if (s + doff >= v) {
-                                    ns = (uint32_t)(s + doff);
-                                    write = true;
-                                }
-                            } else {
-                                const int32_t lh = v - doff;
-                                const int32_t diff = s - lh;
-                                if (diff >= window) {
-                                    ns = (uint32_t)(s + doff);
-                                    write = true;
-                                } else if (diff >= WORD) {
-                                    task = true;
-                                    sl = lh + WORD;
-                                    ns = kPend;
-                                    write = true;
-                                }
-                            }
-                        }
-                    }
+                    const bool win = active && cl == lane;
+                    const int32_t v = (int32_t)(st & kVal);
+                    const int32_t diff = sd - v;  // s - last_hit
+                    const bool live = win && (st & kPend) == 0;
+                    const bool flagged = (st & kFlag) != 0;
+                    // flagged: the hit restarts the diagonal once it is past
+                    // the extension's end; unflagged: a hit beyond the window
+                    // restarts it, one at least a word past the last completes
+                    // a pair
+                    const bool restart = flagged ? diff >= 0 : diff >= window;
+                    const bool task = live && !flagged && !restart && diff >= WORD;
+                    const bool write = live && (restart || task);
+                    const uint32_t ns = task ? kPend : (uint32_t)sd;
+                    const int32_t sl = sd - diff + WORD - doff;  // last_hit + word, subject coordinates
+                    deferred = deferred || (win && !live);
+                    active = active && !win;
+                    if (win) claim[h] = 0xFFFFFFFFu;
                     const unsigned long long tb = __ballot(task);
                     if (tb) {
                         if (task) {
@@ -641,7 +631,8 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                         const uint32_t r =
                             min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
                         const uint32_t start = starts[r];
-                        const uint32_t cnt = present ? starts[r + 1] - start : 0u;
+                        const uint32_t end = starts[r + 1];
+                        const uint32_t cnt = present ? end - start : 0u;
                         const uint32_t incl = wave_inclusive_sum(cnt);
                         const bool fits = incl <= qcap - count;
                         const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
