@@ -41,9 +41,24 @@ namespace bpx {
 
 namespace {
 
-constexpr uint32_t kFlag = 0x80000000u;  // NCBI reset flag: an extension reached past this hit
-constexpr uint32_t kPend = 0x40000000u;  // extension queued, outcome not yet applied
-constexpr uint32_t kVal = 0x3FFFFFFFu;   // last_hit + running diagonal offset
+// Diagonal slot state {value, reset flag, pending}.  Wide slots (u32) hold
+// last_hit + NCBI's running diagonal offset; narrow slots (u16) hold
+// last_hit + window for the current subject only (the table is cleared per
+// subject), so a zero slot reads as "no hit yet" in both.
+template <bool NARROW>
+struct Slot {
+    using T = uint32_t;
+    static constexpr uint32_t kFlag = 0x80000000u;  // NCBI reset flag: an extension reached past this hit
+    static constexpr uint32_t kPend = 0x40000000u;  // extension queued, outcome not yet applied
+    static constexpr uint32_t kVal = 0x3FFFFFFFu;
+};
+template <>
+struct Slot<true> {
+    using T = uint16_t;
+    static constexpr uint32_t kFlag = 0x8000u;
+    static constexpr uint32_t kPend = 0x4000u;
+    static constexpr uint32_t kVal = 0x3FFFu;
+};
 constexpr int32_t kOffsetLimit = 1 << 28;
 constexpr int kMatrixStride = 32;        // LDS score rows padded 28 -> 32
 constexpr uint32_t kCtlBytes = 16;       // workgroup control words at LDS offset 0
@@ -317,8 +332,10 @@ struct QCount {
     uint32_t agg_nseq, agg_ext, agg_saved, agg_passed;
 };
 
-template <int WORD, bool TAB_LDS, bool MULTI>
-__global__ __launch_bounds__(512) void k_scan(ScanParams p) {
+template <int WORD, bool TAB_LDS, bool MULTI, bool NARROW>
+__global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
+    using SlotT = typename Slot<NARROW>::T;
+    constexpr uint32_t kFlag = Slot<NARROW>::kFlag, kPend = Slot<NARROW>::kPend, kVal = Slot<NARROW>::kVal;
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -338,8 +355,9 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
 
     uint8_t* wbase = lds + p.lds_wave + wave * p.wave_stride;
     uint8_t* sbuf = wbase;
-    uint32_t* diag = reinterpret_cast<uint32_t*>(wbase + p.sbuf);
-    uint32_t* claim = diag + p.diag_cap;
+    const uint32_t diag_bytes = (p.diag_cap * (uint32_t)sizeof(SlotT) + 15u) & ~15u;
+    SlotT* diag = reinterpret_cast<SlotT*>(wbase + p.sbuf);
+    uint32_t* claim = reinterpret_cast<uint32_t*>(wbase + p.sbuf + diag_bytes);
     uint32_t* ring = claim + kClaimSlots;
     uint32_t* t_slot = ring + p.qcap;
     uint32_t* t_cj = t_slot + kTaskCap;
@@ -347,7 +365,12 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
     uint32_t* t_sl = t_s + kTaskCap;
     uint32_t* dq = t_sl + kTaskCap;
     QCount* qc = reinterpret_cast<QCount*>(dq + kDefCap);
-    for (uint32_t i = lane; i < p.diag_cap; i += 64) diag[i] = 0;
+    // zero-fills the wave's diagonal table, 16 bytes per lane and store
+    auto clear_diag = [&]() {
+        uint4* d4 = reinterpret_cast<uint4*>(diag);
+        for (uint32_t i = lane; i < diag_bytes / 16u; i += 64) d4[i] = uint4{0u, 0u, 0u, 0u};
+    };
+    clear_diag();
     for (uint32_t i = lane; i < kClaimSlots; i += 64) claim[i] = 0xFFFFFFFFu;
     if (MULTI)
         for (uint32_t i = lane; i < p.max_nq; i += 64) qc[i] = QCount{};
@@ -393,8 +416,13 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
         }
         const uint32_t* table = TAB_LDS ? tab_lds : p.tables + g.table_word;
         const uint2* pvr = reinterpret_cast<const uint2*>(table);
-        const uint32_t* starts = table + 2 * p.ngroups_pv;
-        const uint16_t* entries = reinterpret_cast<const uint16_t*>(starts + g.npresent + 1);
+        // start offsets are u16 (padded to 4 B) for tables of < 65,536
+        // entries, which every LDS-resident table is, else u32
+        const bool narrow_starts = TAB_LDS || (g.flags & kGroupNarrowStarts) != 0;
+        const uint32_t* starts32 = table + 2 * p.ngroups_pv;
+        const uint16_t* starts16 = reinterpret_cast<const uint16_t*>(starts32);
+        const uint16_t* entries = narrow_starts ? starts16 + ((g.npresent + 2u) & ~1u)
+                                                : reinterpret_cast<const uint16_t*>(starts32 + g.npresent + 1);
         const QueryInfo q0 = qinfo[0];
         uint32_t a_nseq = 0, a_ext = 0, a_saved = 0, a_passed = 0;  // single-query aggregates
         unsigned long long a_hits = 0;
@@ -491,7 +519,7 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                         }
                         ntasks += (uint32_t)__popcll(tb);
                     }
-                    if (write) diag[dslot] = ns;
+                    if (write) diag[dslot] = (SlotT)ns;
                     wave_sync();
                 }
                 const unsigned long long db = __ballot(deferred);
@@ -536,8 +564,8 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                     bool rec = false;
                     if (mine) {
                         rec = x.score >= qi.cutoff;
-                        diag[tslot] = x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
-                                                     : (uint32_t)(ts + doff);
+                        diag[tslot] = (SlotT)(x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
+                                                             : (uint32_t)(ts + doff));
                         if (MULTI) {
                             atomicAdd(&qc[tj].ext, 1u);
                             if (rec) atomicAdd(&qc[tj].saved, 1u);
@@ -630,8 +658,14 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                         // masked out
                         const uint32_t r =
                             min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
-                        const uint32_t start = starts[r];
-                        const uint32_t end = starts[r + 1];
+                        uint32_t start, end;
+                        if (narrow_starts) {
+                            start = starts16[r];
+                            end = starts16[r + 1];
+                        } else {
+                            start = starts32[r];
+                            end = starts32[r + 1];
+                        }
                         const uint32_t cnt = present ? end - start : 0u;
                         const uint32_t incl = wave_inclusive_sum(cnt);
                         const bool fits = incl <= qcap - count;
@@ -714,9 +748,10 @@ __global__ __launch_bounds__(512) void k_scan(ScanParams p) {
                 wave_sync();
             }
             // Blast_ExtendWordExit: a fresh diagonal range for the next
-            // subject; clear the table before the offset field runs out.
-            if (doff >= kOffsetLimit) {
-                for (uint32_t i = lane; i < p.diag_cap; i += 64) diag[i] = 0;
+            // subject.  Narrow slots: clear the table; wide slots: advance
+            // the running offset, clearing the table before it runs out.
+            if (NARROW || doff >= kOffsetLimit) {
+                clear_diag();
                 doff = window;
                 wave_sync();
             } else {
@@ -864,12 +899,12 @@ __global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
 }  // namespace
 
 LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap, uint32_t qcap,
-                 uint32_t sbuf, int waves, bool table_in_lds) {
+                 uint32_t sbuf, int waves, bool table_in_lds, bool narrow) {
     auto up16 = [](uint32_t v) { return (v + 15u) & ~15u; };
     LdsPlan pl{};
     pl.max_nq = max_nq < 1 ? 1 : max_nq;
     pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
-    pl.wave_stride = up16(sbuf) + up16(diag_cap * 4) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 + kDefCap * 4 +
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 + kDefCap * 4 +
                      (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
     uint32_t at = kCtlBytes;
     pl.table = at;
@@ -889,19 +924,21 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     return pl;
 }
 
-hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, int waves, int groups,
+hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
                        hipStream_t stream) {
+    if (waves < 1 || waves > kMaxWaves) return hipErrorInvalidValue;
     const dim3 grid((unsigned)groups), block((unsigned)(64 * waves));
     void (*fn)(ScanParams) = nullptr;
-#define BPX_PICK(W)                                                                                   \
-    fn = table_in_lds ? (multi ? k_scan<W, true, true> : k_scan<W, true, false>)                      \
-                      : (multi ? k_scan<W, false, true> : k_scan<W, false, false>)
-    if (p.word == 3)
-        BPX_PICK(3);
-    else if (p.word == 2)
-        BPX_PICK(2);
-    else
+#define BPX_PICK(W, N)                                                                                \
+    fn = table_in_lds ? (multi ? k_scan<W, true, true, N> : k_scan<W, true, false, N>)                \
+                      : (multi ? k_scan<W, false, true, N> : k_scan<W, false, false, N>)
+    if (p.word == 3) {
+        if (narrow) BPX_PICK(3, true); else BPX_PICK(3, false);
+    } else if (p.word == 2) {
+        if (narrow) BPX_PICK(2, true); else BPX_PICK(2, false);
+    } else {
         return hipErrorInvalidValue;
+    }
 #undef BPX_PICK
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)p.lds_total);

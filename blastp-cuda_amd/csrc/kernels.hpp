@@ -33,7 +33,10 @@ struct GroupInfo {
     uint32_t qinfo_first;  // first QueryInfo of the group
     uint32_t nq;           // queries in the group
     uint32_t diag_slots;   // sum of the members' diagonal-table lengths
+    uint32_t flags;        // kGroupNarrowStarts: u16 start offsets
+    uint32_t pad[3];
 };
+constexpr uint32_t kGroupNarrowStarts = 1;
 
 // A unit of dynamic work: one group against a run of the subject order.
 struct WorkItem {
@@ -88,7 +91,7 @@ struct ScanParams {
     uint32_t table_cap_words;   // LDS table region (words); 0 = tables read from HBM
     uint32_t qres_cap;          // LDS bytes for concatenated residues / query map
     uint32_t max_nq;            // LDS slots for QueryInfo / per-query counters
-    uint32_t diag_cap;          // diagonal slots per wave
+    uint32_t diag_cap;          // diagonal slots per wave (u16 slots in narrow launches)
     uint32_t qcap;              // hit ring entries per wave (power of two)
     uint32_t sbuf;              // subject buffer bytes per wave (power of two)
 };
@@ -102,10 +105,16 @@ constexpr uint32_t kClaimSlots = 256;  // hashed claim table entries per wave
 constexpr uint32_t kTaskCap = 128;     // queued extensions per wave
 constexpr uint32_t kDefCap = 128;      // deferred hits per wave (hits on pending slots)
 
-LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
-                 uint32_t qcap, uint32_t sbuf, int waves, bool table_in_lds);
+// Narrow launches keep 16-bit diagonal slots and clear the table for every
+// subject; they take subjects with length + window < kNarrowLimit.  Wide
+// launches keep 32-bit slots with NCBI's running diagonal offset.
+constexpr uint32_t kNarrowLimit = 1u << 14;
+constexpr int kMaxWaves = 16;
 
-hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, int waves, int groups,
+LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
+                 uint32_t qcap, uint32_t sbuf, int waves, bool table_in_lds, bool narrow);
+
+hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
                        hipStream_t stream);
 
 // Compaction: counts -> offsets (in place) -> records in (query, subject,

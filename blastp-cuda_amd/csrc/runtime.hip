@@ -44,7 +44,7 @@ int fail(int code, const char* fmt, ...) {
 
 constexpr uint32_t kArenaAlign = 16;   // subject starts in the arena
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
-constexpr int kMaxClasses = 4;         // (table in LDS?) x (multi-query groups?)
+constexpr int kMaxClasses = 8;         // (table in LDS?) x (multi-query groups?) x (narrow slots?)
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
@@ -208,7 +208,7 @@ struct bpx_result {
     float kernel_ms = 0.f, device_ms = 0.f;
     // launch plan
     struct Class {
-        bool tab_lds = false, multi = false;
+        bool tab_lds = false, multi = false, narrow = true;
         int waves = 0;
         bpx::LdsPlan plan{};
         uint32_t item_first = 0, nitems = 0;
@@ -290,6 +290,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
         gi.table_word = (uint32_t)tables.size();
         gi.table_words = (uint32_t)blob.size();
         gi.npresent = t.npresent;
+        gi.flags = t.narrow_starts() ? bpx::kGroupNarrowStarts : 0u;
         tables.insert(tables.end(), blob.begin(), blob.end());
         set.max_cell.push_back(t.max_cell);
         set.groups.push_back(gi);
@@ -341,11 +342,13 @@ bpx_launch resolve_launch(const bpx_launch* in) {
 }
 
 // Splits the set's groups into launch classes and lays out LDS for each.
-int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_launch* user) {
+// Subjects [0, nwide) of the processing order (the longest, length + window
+// >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
+int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
     bpx_ctx* ctx = r->ctx;
     const bpx_launch L = resolve_launch(user);
     const bool auto_waves = !user || user->waves_per_group <= 0;
-    const int want_waves = auto_waves ? 8 : std::min(8, std::max(1, user->waves_per_group));
+    const int user_waves = auto_waves ? 0 : std::min(bpx::kMaxWaves, std::max(1, user->waves_per_group));
     uint32_t maxcell = 0;
     for (uint32_t c : set.max_cell) maxcell = std::max(maxcell, c);
     uint32_t qc = L.queue_capacity > 0 ? (uint32_t)L.queue_capacity : 256u;
@@ -355,16 +358,23 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_laun
     r->qcap = qcap;
     const uint32_t sbuf = (uint32_t)L.subject_buffer;
     const uint32_t limit = (uint32_t)ctx->lds_limit;
+    const bool any_narrow = nwide < count, any_wide = nwide > 0;
 
     struct Acc {
         std::vector<uint32_t> groups;
         uint32_t table = 0, qres = 0, nq = 1, diag = 0;
     };
-    auto plan_of = [&](const Acc& a, bool tab_lds, int waves) {
-        return bpx::plan_lds(a.table, a.qres, a.nq, a.diag, qcap, sbuf, waves, tab_lds);
+    auto plan_of = [&](const Acc& a, bool tab_lds, int waves, bool narrow) {
+        return bpx::plan_lds(a.table, a.qres, a.nq, a.diag, qcap, sbuf, waves, tab_lds, narrow);
     };
-    Acc acc[kMaxClasses];  // index: tab_lds * 2 + multi
-    // groups by table size: the LDS class takes the smaller tables first
+    // does one wave of every slot width this scan needs fit with the table in LDS?
+    auto fits_lds = [&](const Acc& a) {
+        return (!any_narrow || plan_of(a, true, 1, true).total <= limit) &&
+               (!any_wide || plan_of(a, true, 1, false).total <= limit);
+    };
+    Acc acc[4];  // index: tab_lds * 2 + multi
+    // groups by table size: the LDS class takes the smaller tables first;
+    // tables with u32 start offsets stay in HBM
     std::vector<uint32_t> gorder(set.groups.size());
     std::iota(gorder.begin(), gorder.end(), 0u);
     std::stable_sort(gorder.begin(), gorder.end(),
@@ -378,8 +388,7 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_laun
         t.qres = std::max(t.qres, g.qres_len);
         t.nq = std::max(t.nq, g.nq);
         t.diag = std::max(t.diag, g.diag_slots);
-        // keep the group in the LDS class if at least one wave still fits
-        if (plan_of(t, true, 1).total <= limit) {
+        if ((g.flags & bpx::kGroupNarrowStarts) && fits_lds(t)) {
             acc[2 + multi] = t;
         } else {
             Acc& h = acc[multi];
@@ -395,32 +404,44 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, const bpx_laun
         L.subjects_per_item > 0
             ? (uint32_t)L.subjects_per_item
             : std::max<uint32_t>(64u, std::min<uint32_t>(4096u, count / (uint32_t)std::max(1, ctx->num_cus * 4) + 1));
-    for (int cls = kMaxClasses - 1; cls >= 0; --cls) {
+    static const int kAutoWaves[] = {12, 8, 4, 2, 1};
+    for (int cls = 3; cls >= 0; --cls) {
         const Acc& a = acc[cls];
         if (a.groups.empty()) continue;
-        bpx_result::Class c;
-        c.tab_lds = cls >= 2;
-        c.multi = (cls & 1) != 0;
-        int waves = want_waves;
-        bpx::LdsPlan pl = plan_of(a, c.tab_lds, waves);
-        while (auto_waves && waves > 1 && pl.total > limit) {
-            waves /= 2;
-            pl = plan_of(a, c.tab_lds, waves);
+        for (int w = 0; w < 2; ++w) {
+            const bool narrow = w == 1;
+            const uint32_t lo = narrow ? nwide : 0u, hi = narrow ? count : nwide;
+            if (lo >= hi) continue;
+            bpx_result::Class c;
+            c.tab_lds = cls >= 2;
+            c.multi = (cls & 1) != 0;
+            c.narrow = narrow;
+            int waves = user_waves;
+            bpx::LdsPlan pl{};
+            if (auto_waves) {
+                for (int cand : kAutoWaves) {
+                    waves = cand;
+                    pl = plan_of(a, c.tab_lds, waves, narrow);
+                    if (pl.total <= limit) break;
+                }
+            } else {
+                pl = plan_of(a, c.tab_lds, waves, narrow);
+            }
+            r->lds_need = std::max(r->lds_need, pl.total);
+            if (pl.total > limit)
+                return fail(BPX_E_BUDGET,
+                            "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
+                            "use fewer waves per group or a smaller subject buffer",
+                            pl.total, limit, a.diag, waves, sbuf);
+            c.waves = waves;
+            c.plan = pl;
+            c.diag_cap = std::max<uint32_t>(a.diag, 1);
+            c.item_first = (uint32_t)r->items.size();
+            for (uint32_t gi : a.groups)
+                for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
+            c.nitems = (uint32_t)r->items.size() - c.item_first;
+            r->classes.push_back(c);
         }
-        r->lds_need = std::max(r->lds_need, pl.total);
-        if (pl.total > limit)
-            return fail(BPX_E_BUDGET,
-                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
-                        "use fewer waves per group or a smaller subject buffer",
-                        pl.total, limit, a.diag, waves, sbuf);
-        c.waves = waves;
-        c.plan = pl;
-        c.diag_cap = std::max<uint32_t>(a.diag, 1);
-        c.item_first = (uint32_t)r->items.size();
-        for (uint32_t gi : a.groups)
-            for (uint32_t f = 0; f < count; f += R) r->items.push_back({gi, f, std::min(R, count - f), 0});
-        c.nitems = (uint32_t)r->items.size() - c.item_first;
-        r->classes.push_back(c);
     }
     r->launch = L;
     return BPX_OK;
@@ -566,7 +587,7 @@ int enqueue(bpx_result* r, const uint32_t* order) {
         sp.diag_cap = c.diag_cap;
         sp.qcap = r->qcap;
         sp.sbuf = (uint32_t)r->launch.subject_buffer;
-        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.waves, groups, st));
+        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st));
         seg_at += (uint64_t)groups * c.waves * r->seg_cap;
     }
     HIP_TRY(hipEventRecord(r->ev[1], st));
@@ -598,6 +619,17 @@ int enqueue(bpx_result* r, const uint32_t* order) {
     return BPX_OK;
 }
 
+// Subjects of [first, first + count) too long for narrow diagonal slots.
+// They lead the longest-first processing order.
+uint32_t count_wide(const bpx_db* db, uint32_t first, uint32_t count, int32_t window) {
+    uint32_t n = 0;
+    for (uint32_t id : db->h_order) {
+        if ((uint64_t)db->h_len[id] + (uint64_t)std::max(window, 0) < bpx::kNarrowLimit) break;
+        if (id >= first && id - first < count) ++n;
+    }
+    return n;
+}
+
 int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t first, uint32_t count,
                const bpx_launch* launch, uint32_t out_flags, bpx_result** out) {
     if (!ctx || !db || !out) return fail(BPX_E_ARG, "NULL handle");
@@ -613,7 +645,7 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     r->first = first;
     r->count = count;
     r->flags = out_flags;
-    int rc = plan_scan(r, set, count, launch);
+    int rc = plan_scan(r, set, count, count_wide(db, first, count, set.window), launch);
     if (rc) {
         delete r;
         return rc;
@@ -836,7 +868,7 @@ int bpx_query_info(const bpx_query* q, int64_t* table_bytes, int32_t* lds_reside
         bpx_result tmp;
         tmp.ctx = q->set.ctx;
         *lds_resident = 0;
-        if (plan_scan(&tmp, q->set, 1, nullptr) == BPX_OK && !tmp.classes.empty())
+        if (plan_scan(&tmp, q->set, 1, 0, nullptr) == BPX_OK && !tmp.classes.empty())
             *lds_resident = tmp.classes[0].tab_lds ? 1 : 0;
     }
     if (num_present_words) *num_present_words = q->npresent;
@@ -908,7 +940,7 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
     if ((uint64_t)first + count > db->n) return fail(BPX_E_ARG, "subject range outside the database");
     bpx_result tmp;
     tmp.ctx = ctx;
-    int rc = plan_scan(&tmp, q->set, count, launch);
+    int rc = plan_scan(&tmp, q->set, count, count_wide(db, first, count, q->set.window), launch);
     const uint32_t lds = tmp.lds_need;
     const uint64_t waves = std::max<uint64_t>(total_waves(&tmp), 1);
     const uint64_t seg = std::max<uint64_t>(64, ((uint64_t)count / 8 + 65536) / waves + 1) * waves;

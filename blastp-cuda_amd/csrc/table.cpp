@@ -168,9 +168,12 @@ bool build_table_multi(const QuerySpan* queries, int32_t nq, const int8_t* M,
     return true;
 }
 
+bool HostTable::narrow_starts() const { return entries.size() <= 0xFFFFu; }
+
 uint64_t HostTable::packed_bytes() const {
     const uint64_t e = ((uint64_t)entries.size() * 2 + 3) & ~3ull;
-    return 8ull * ngroups + 4ull * (npresent + 1) + e;
+    const uint64_t st = narrow_starts() ? (2ull * (npresent + 1) + 3) & ~3ull : 4ull * (npresent + 1);
+    return 8ull * ngroups + st + e;
 }
 
 std::vector<uint32_t> HostTable::pack() const {
@@ -178,8 +181,14 @@ std::vector<uint32_t> HostTable::pack() const {
     uint8_t* b = reinterpret_cast<uint8_t*>(blob.data());
     std::memcpy(b, groups.data(), 8ull * ngroups);
     b += 8ull * ngroups;
-    std::memcpy(b, starts.data(), 4ull * (npresent + 1));
-    b += 4ull * (npresent + 1);
+    if (narrow_starts()) {
+        uint16_t* s16 = reinterpret_cast<uint16_t*>(b);
+        for (uint32_t r = 0; r <= npresent; ++r) s16[r] = (uint16_t)starts[r];
+        b += (2ull * (npresent + 1) + 3) & ~3ull;
+    } else {
+        std::memcpy(b, starts.data(), 4ull * (npresent + 1));
+        b += 4ull * (npresent + 1);
+    }
     if (!entries.empty()) std::memcpy(b, entries.data(), 2ull * entries.size());
     return blob;
 }

@@ -36,7 +36,10 @@ struct HostTable {
     uint32_t max_cell = 0;
 
     // Serialises the table into one word-aligned blob:
-    // [groups (8*ngroups B)][starts (4*(npresent+1) B)][entries (2*n B, padded to 4)]
+    // [groups (8*ngroups B)][starts][entries (2*n B, padded to 4)], where the
+    // starts are u16 (padded to 4 B) when the table has at most 65,535
+    // entries and u32 otherwise.
+    bool narrow_starts() const;
     std::vector<uint32_t> pack() const;
     uint64_t packed_bytes() const;
 };

@@ -103,7 +103,7 @@ typedef struct bpx_ungapped_stats {
  * -gpu_threads, cmdline_flags.cpp:218-222, re-expressed for gfx950).  Any
  * field left 0 takes the library default or the autotuned value. */
 typedef struct bpx_launch {
-    int32_t waves_per_group;    /* wavefronts per workgroup (1..8) */
+    int32_t waves_per_group;    /* wavefronts per workgroup (1..16); 0 = the most that fit */
     int32_t groups_per_cu;      /* resident workgroups per CU (grid = CUs x this) */
     int32_t subjects_per_item;  /* subjects per dynamic work item */
     int32_t queue_capacity;     /* per-wave LDS hit ring entries (power of two) */
