@@ -358,12 +358,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     const uint32_t diag_bytes = (p.diag_cap * (uint32_t)sizeof(SlotT) + 15u) & ~15u;
     SlotT* diag = reinterpret_cast<SlotT*>(wbase + p.sbuf);
     uint32_t* claim = reinterpret_cast<uint32_t*>(wbase + p.sbuf + diag_bytes);
-    uint32_t* ring = claim + kClaimSlots;
-    uint32_t* t_slot = ring + p.qcap;
-    uint32_t* t_cj = t_slot + kTaskCap;
-    uint32_t* t_s = t_cj + kTaskCap;
-    uint32_t* t_sl = t_s + kTaskCap;
-    uint32_t* dq = t_sl + kTaskCap;
+    uint32_t* ring = claim + p.claim_slots;
+    // queued extension: {slot | second hit's group position << 16, subject
+    // offset}; the first hit stays in the pending slot's value bits
+    uint32_t* t_sc = ring + p.qcap;
+    uint32_t* t_s = t_sc + kTaskCap;
+    uint32_t* dq = t_s + kTaskCap;
     QCount* qc = reinterpret_cast<QCount*>(dq + kDefCap);
     // zero-fills the wave's diagonal table, 16 bytes per lane and store
     auto clear_diag = [&]() {
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
         for (uint32_t i = lane; i < diag_bytes / 16u; i += 64) d4[i] = uint4{0u, 0u, 0u, 0u};
     };
     clear_diag();
-    for (uint32_t i = lane; i < kClaimSlots; i += 64) claim[i] = 0xFFFFFFFFu;
+    for (uint32_t i = lane; i < p.claim_slots; i += 64) claim[i] = 0xFFFFFFFFu;
     if (MULTI)
         for (uint32_t i = lane; i < p.max_nq; i += 64) qc[i] = QCount{};
 
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     if (active && fresh) atomicAdd(&qc[j].hits, 1u);
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
-                const uint32_t h = dslot & (kClaimSlots - 1);
+                const uint32_t h = dslot & (p.claim_slots - 1);
                 bool deferred = false;
                 const int32_t sd = s + doff;
                 while (__ballot(active) != 0) {
@@ -503,8 +503,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     const bool restart = flagged ? diff >= 0 : diff >= window;
                     const bool task = live && !flagged && !restart && diff >= WORD;
                     const bool write = live && (restart || task);
-                    const uint32_t ns = task ? kPend : (uint32_t)sd;
-                    const int32_t sl = sd - diff + WORD - doff;  // last_hit + word, subject coordinates
+                    const uint32_t ns = task ? (st | kPend) : (uint32_t)sd;
                     deferred = deferred || (win && !live);
                     active = active && !win;
                     if (win) claim[h] = 0xFFFFFFFFu;
@@ -512,10 +511,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     if (tb) {
                         if (task) {
                             const uint32_t idx = ntasks + rank_below(tb);
-                            t_slot[idx] = dslot;
-                            t_cj[idx] = ((uint32_t)c << 16) | j;
+                            t_sc[idx] = dslot | ((uint32_t)c << 16);
                             t_s[idx] = (uint32_t)s;
-                            t_sl[idx] = (uint32_t)sl;
                         }
                         ntasks += (uint32_t)__popcll(tb);
                     }
@@ -541,12 +538,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     const bool tvalid = t < nt;
                     const bool mine = coop ? (tvalid && (lane & 15u) == 0) : tvalid;
                     const uint32_t ti = min(t, kTaskCap - 1);
-                    const uint32_t tslot = t_slot[ti];
-                    const uint32_t cj = t_cj[ti];
-                    int32_t tc = (int32_t)(cj >> 16);
-                    const uint32_t tj = cj & 0xFFFFu;
+                    const uint32_t sc = tvalid ? t_sc[ti] : 0u;  // slot 0 / position 0 for idle lanes
+                    const uint32_t tslot = sc & 0xFFFFu;
+                    int32_t tc = (int32_t)(sc >> 16);
+                    const uint32_t tj = MULTI ? (uint32_t)qmap[tc] : 0u;
                     const int32_t ts = (int32_t)t_s[ti];
-                    const int32_t tsl = (int32_t)t_sl[ti];
+                    // left boundary: the first hit (kept in the slot) + word
+                    const int32_t tsl = (int32_t)(diag[tslot] & kVal) - doff + WORD;
                     const QueryInfo qi = MULTI ? qinfo[tj] : q0;
                     TwoHit x{};
                     if (coop) {
@@ -899,12 +897,14 @@ __global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
 }  // namespace
 
 LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap, uint32_t qcap,
-                 uint32_t sbuf, int waves, bool table_in_lds, bool narrow) {
+                 uint32_t sbuf, uint32_t claim_slots, int waves, bool table_in_lds, bool narrow) {
     auto up16 = [](uint32_t v) { return (v + 15u) & ~15u; };
     LdsPlan pl{};
     pl.max_nq = max_nq < 1 ? 1 : max_nq;
+    pl.qcap = qcap;
+    pl.claim_slots = claim_slots;
     pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
-    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + kClaimSlots * 4 + qcap * 4 + kTaskCap * 16 + kDefCap * 4 +
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + claim_slots * 4 + qcap * 4 + kTaskCap * 8 + kDefCap * 4 +
                      (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
     uint32_t at = kCtlBytes;
     pl.table = at;

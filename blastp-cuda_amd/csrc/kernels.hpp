@@ -93,15 +93,16 @@ struct ScanParams {
     uint32_t max_nq;            // LDS slots for QueryInfo / per-query counters
     uint32_t diag_cap;          // diagonal slots per wave (u16 slots in narrow launches)
     uint32_t qcap;              // hit ring entries per wave (power of two)
+    uint32_t claim_slots;       // hashed claim table entries per wave (power of two, >= 64)
     uint32_t sbuf;              // subject buffer bytes per wave (power of two)
 };
 
 struct LdsPlan {
     uint32_t table, qres, qmap, qinfo, matrix, wave, wave_stride, total;
     uint32_t table_cap_words, qres_cap, max_nq;
+    uint32_t qcap, claim_slots;
 };
 
-constexpr uint32_t kClaimSlots = 256;  // hashed claim table entries per wave
 constexpr uint32_t kTaskCap = 128;     // queued extensions per wave
 constexpr uint32_t kDefCap = 128;      // deferred hits per wave (hits on pending slots)
 
@@ -112,7 +113,7 @@ constexpr uint32_t kNarrowLimit = 1u << 14;
 constexpr int kMaxWaves = 16;
 
 LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
-                 uint32_t qcap, uint32_t sbuf, int waves, bool table_in_lds, bool narrow);
+                 uint32_t qcap, uint32_t sbuf, uint32_t claim_slots, int waves, bool table_in_lds, bool narrow);
 
 hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
                        hipStream_t stream);

@@ -216,7 +216,6 @@ struct bpx_result {
     };
     std::vector<Class> classes;
     std::vector<bpx::WorkItem> items;
-    uint32_t qcap = 0;
     uint32_t lds_need = 0;  // largest per-workgroup LDS plan, including one that did not fit
 };
 
@@ -351,11 +350,17 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
     const int user_waves = auto_waves ? 0 : std::min(bpx::kMaxWaves, std::max(1, user->waves_per_group));
     uint32_t maxcell = 0;
     for (uint32_t c : set.max_cell) maxcell = std::max(maxcell, c);
-    uint32_t qc = L.queue_capacity > 0 ? (uint32_t)L.queue_capacity : 256u;
-    qc = std::max<uint32_t>(qc, std::max<uint32_t>(128u, maxcell + 64u));
-    uint32_t qcap = 1;
-    while (qcap < qc) qcap <<= 1;
-    r->qcap = qcap;
+    // hit ring: the requested capacity, else 256 entries, shrunk to the
+    // smallest that holds a batch plus the largest cell when LDS is short
+    auto pow2_at_least = [](uint32_t v) {
+        uint32_t n = 1;
+        while (n < v) n <<= 1;
+        return n;
+    };
+    const uint32_t qmin = pow2_at_least(std::max<uint32_t>(128u, maxcell + 64u));
+    const uint32_t qdef = L.queue_capacity > 0 ? std::max(qmin, pow2_at_least((uint32_t)L.queue_capacity))
+                                               : std::max(qmin, 256u);
+    const uint32_t qalt = L.queue_capacity > 0 ? qdef : qmin;
     const uint32_t sbuf = (uint32_t)L.subject_buffer;
     const uint32_t limit = (uint32_t)ctx->lds_limit;
     const bool any_narrow = nwide < count, any_wide = nwide > 0;
@@ -364,8 +369,16 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
         std::vector<uint32_t> groups;
         uint32_t table = 0, qres = 0, nq = 1, diag = 0;
     };
+    // Per-wave buffers for `waves` waves: the first of (claim table, ring)
+    // = (256, default), (256, small), (128, default), (128, small) that fits.
     auto plan_of = [&](const Acc& a, bool tab_lds, int waves, bool narrow) {
-        return bpx::plan_lds(a.table, a.qres, a.nq, a.diag, qcap, sbuf, waves, tab_lds, narrow);
+        const uint32_t claims[4] = {256u, 256u, 128u, 128u}, rings[4] = {qdef, qalt, qdef, qalt};
+        bpx::LdsPlan pl{};
+        for (int k = 0; k < 4; ++k) {
+            pl = bpx::plan_lds(a.table, a.qres, a.nq, a.diag, rings[k], sbuf, claims[k], waves, tab_lds, narrow);
+            if (pl.total <= limit) break;
+        }
+        return pl;
     };
     // does one wave of every slot width this scan needs fit with the table in LDS?
     auto fits_lds = [&](const Acc& a) {
@@ -404,7 +417,7 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
         L.subjects_per_item > 0
             ? (uint32_t)L.subjects_per_item
             : std::max<uint32_t>(64u, std::min<uint32_t>(4096u, count / (uint32_t)std::max(1, ctx->num_cus * 4) + 1));
-    static const int kAutoWaves[] = {12, 8, 4, 2, 1};
+    static const int kAutoWaves[] = {16, 12, 8, 4, 2, 1};
     for (int cls = 3; cls >= 0; --cls) {
         const Acc& a = acc[cls];
         if (a.groups.empty()) continue;
@@ -412,6 +425,9 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             const bool narrow = w == 1;
             const uint32_t lo = narrow ? nwide : 0u, hi = narrow ? count : nwide;
             if (lo >= hi) continue;
+            if (a.diag > 65536u)
+                return fail(BPX_E_BUDGET, "a scan group needs %u diagonal slots (at most 65536); "
+                            "use smaller query groups", a.diag);
             bpx_result::Class c;
             c.tab_lds = cls >= 2;
             c.multi = (cls & 1) != 0;
@@ -585,7 +601,8 @@ int enqueue(bpx_result* r, const uint32_t* order) {
         sp.qres_cap = c.plan.qres_cap;
         sp.max_nq = c.plan.max_nq;
         sp.diag_cap = c.diag_cap;
-        sp.qcap = r->qcap;
+        sp.qcap = c.plan.qcap;
+        sp.claim_slots = c.plan.claim_slots;
         sp.sbuf = (uint32_t)r->launch.subject_buffer;
         HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st));
         seg_at += (uint64_t)groups * c.waves * r->seg_cap;
