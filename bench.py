@@ -131,7 +131,7 @@ def main():
     achieved = bytes_alg / (kern / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_scan<3,true>", "kernel_ms_mean": round(kern, 4),
+                "kernel": "k_scan<3,true,false,true>", "kernel_ms_mean": round(kern, 4),
                 "kernel_ms_p50": round(float(np.percentile(kernel_ms, 50)), 4),
                 "kernel_ms_p99": round(float(np.percentile(kernel_ms, 99)), 4),
                 "bytes_alg": int(bytes_alg)}

@@ -46,6 +46,7 @@ constexpr uint32_t kArenaAlign = 16;   // subject starts in the arena
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
 constexpr int kMaxClasses = 8;         // (table in LDS?) x (multi-query groups?) x (narrow slots?)
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
+constexpr uint32_t kSegFloor = 256;    // smallest per-wave record segment of a first scan
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
 // do not call hipMalloc/hipFree.
@@ -675,7 +676,7 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     const uint64_t waves = std::max<uint64_t>(total_waves(r), 1);
     const uint64_t pairs = (uint64_t)set.nq * count;
     const uint32_t seg_cap =
-        (uint32_t)std::max<uint64_t>(std::max<uint64_t>(64, ctx->seg_hint), (pairs / 8 + 65536) / waves + 1);
+        (uint32_t)std::max<uint64_t>(std::max<uint64_t>(kSegFloor, ctx->seg_hint), (pairs / 8 + 65536) / waves + 1);
     rc = alloc_buffers(r, seg_cap);
     if (rc) {
         bpx_result_destroy(r);
@@ -960,7 +961,7 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
     int rc = plan_scan(&tmp, q->set, count, count_wide(db, first, count, q->set.window), launch);
     const uint32_t lds = tmp.lds_need;
     const uint64_t waves = std::max<uint64_t>(total_waves(&tmp), 1);
-    const uint64_t seg = std::max<uint64_t>(64, ((uint64_t)count / 8 + 65536) / waves + 1) * waves;
+    const uint64_t seg = std::max<uint64_t>(kSegFloor, ((uint64_t)count * q->set.nq / 8 + 65536) / waves + 1) * waves;
     const uint64_t need = db->arena_bytes + (uint64_t)db->n * 16 + q->set.table_bytes + (uint64_t)count * 24 +
                           seg * (sizeof(bpx::SegRecord) + sizeof(bpx_init_hsp) + 8);
     if (device_bytes) *device_bytes = (size_t)need;
