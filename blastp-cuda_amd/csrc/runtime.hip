@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -44,7 +45,7 @@ int fail(int code, const char* fmt, ...) {
 
 constexpr uint32_t kArenaAlign = 16;   // subject starts in the arena
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
-constexpr int kMaxClasses = 8;         // (table in LDS?) x (multi-query groups?) x (narrow slots?)
+constexpr int kMaxClasses = 64;        // launches per scan: (table placement, multi, waves) bins x slot width
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 constexpr uint32_t kSegFloor = 256;    // smallest per-wave record segment of a first scan
 
@@ -369,6 +370,13 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
     struct Acc {
         std::vector<uint32_t> groups;
         uint32_t table = 0, qres = 0, nq = 1, diag = 0;
+        void add(uint32_t gi, const bpx::GroupInfo& g) {
+            groups.push_back(gi);
+            table = std::max(table, g.table_words * 4);
+            qres = std::max(qres, g.qres_len);
+            nq = std::max(nq, g.nq);
+            diag = std::max(diag, g.diag_slots);
+        }
     };
     // Per-wave buffers for `waves` waves: the first of (claim table, ring)
     // = (256, default), (256, small), (128, default), (128, small) that fits.
@@ -381,36 +389,43 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
         }
         return pl;
     };
-    // does one wave of every slot width this scan needs fit with the table in LDS?
-    auto fits_lds = [&](const Acc& a) {
-        return (!any_narrow || plan_of(a, true, 1, true).total <= limit) &&
-               (!any_wide || plan_of(a, true, 1, false).total <= limit);
-    };
-    Acc acc[4];  // index: tab_lds * 2 + multi
-    // groups by table size: the LDS class takes the smaller tables first;
-    // tables with u32 start offsets stay in HBM
-    std::vector<uint32_t> gorder(set.groups.size());
-    std::iota(gorder.begin(), gorder.end(), 0u);
-    std::stable_sort(gorder.begin(), gorder.end(),
-                     [&](uint32_t a, uint32_t b) { return set.groups[a].table_words < set.groups[b].table_words; });
-    for (uint32_t gi : gorder) {
-        const bpx::GroupInfo& g = set.groups[gi];
-        const int multi = g.nq > 1 ? 1 : 0;
-        Acc t = acc[2 + multi];
-        t.groups.push_back(gi);
-        t.table = std::max(t.table, g.table_words * 4);
-        t.qres = std::max(t.qres, g.qres_len);
-        t.nq = std::max(t.nq, g.nq);
-        t.diag = std::max(t.diag, g.diag_slots);
-        if ((g.flags & bpx::kGroupNarrowStarts) && fits_lds(t)) {
-            acc[2 + multi] = t;
-        } else {
-            Acc& h = acc[multi];
-            h.groups.push_back(gi);
-            h.qres = std::max(h.qres, g.qres_len);
-            h.nq = std::max(h.nq, g.nq);
-            h.diag = std::max(h.diag, g.diag_slots);
+    static const int kAutoWaves[] = {16, 12, 8, 4, 2, 1};
+    const int* cands = auto_waves ? kAutoWaves : &user_waves;
+    const int ncands = auto_waves ? (int)(sizeof kAutoWaves / sizeof kAutoWaves[0]) : 1;
+    // the most waves (of the candidates) at which `a` fits for every slot
+    // width this scan needs; 0 if none
+    auto best_waves = [&](const Acc& a, bool tab_lds, int below) {
+        for (int k = 0; k < ncands; ++k) {
+            const int w = cands[k];
+            if (w > below) continue;
+            if ((!any_narrow || plan_of(a, tab_lds, w, true).total <= limit) &&
+                (!any_wide || plan_of(a, tab_lds, w, false).total <= limit))
+                return w;
         }
+        return 0;
+    };
+    // Bin the groups: each goes where it runs with the most waves -- its
+    // table in LDS when that keeps at least kLdsMinWaves waves (or as many as
+    // the HBM table would), else its table in HBM.  Groups of one bin share
+    // a launch.  Key: (table in LDS, multi-query, waves).
+    constexpr int kLdsMinWaves = 8;
+    std::map<std::tuple<int, int, int>, Acc> bins;
+    for (uint32_t gi = 0; gi < set.groups.size(); ++gi) {
+        const bpx::GroupInfo& g = set.groups[gi];
+        Acc one;
+        one.add(gi, g);
+        const int wl = (g.flags & bpx::kGroupNarrowStarts) ? best_waves(one, true, bpx::kMaxWaves) : 0;
+        const int wh = best_waves(one, false, bpx::kMaxWaves);
+        if (wl == 0 && wh == 0) {
+            const bpx::LdsPlan pl = plan_of(one, false, cands[ncands - 1], any_narrow);
+            r->lds_need = std::max(r->lds_need, pl.total);
+            return fail(BPX_E_BUDGET,
+                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
+                        "use fewer waves per group or a smaller subject buffer",
+                        pl.total, limit, one.diag, cands[ncands - 1], sbuf);
+        }
+        const bool lds = wl > 0 && (wl >= kLdsMinWaves || wl >= wh);
+        bins[std::make_tuple(lds ? 1 : 0, g.nq > 1 ? 1 : 0, lds ? wl : wh)].add(gi, g);
     }
     r->classes.clear();
     r->items.clear();
@@ -418,40 +433,42 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
         L.subjects_per_item > 0
             ? (uint32_t)L.subjects_per_item
             : std::max<uint32_t>(64u, std::min<uint32_t>(4096u, count / (uint32_t)std::max(1, ctx->num_cus * 4) + 1));
-    static const int kAutoWaves[] = {16, 12, 8, 4, 2, 1};
-    for (int cls = 3; cls >= 0; --cls) {
-        const Acc& a = acc[cls];
-        if (a.groups.empty()) continue;
+    // LDS bins first, then by waves, most first
+    std::vector<std::pair<std::tuple<int, int, int>, Acc*>> order;
+    for (auto& kv : bins) order.push_back({kv.first, &kv.second});
+    std::sort(order.begin(), order.end(), [](const auto& x, const auto& y) {
+        if (std::get<0>(x.first) != std::get<0>(y.first)) return std::get<0>(x.first) > std::get<0>(y.first);
+        if (std::get<2>(x.first) != std::get<2>(y.first)) return std::get<2>(x.first) > std::get<2>(y.first);
+        return std::get<1>(x.first) < std::get<1>(y.first);
+    });
+    for (const auto& ob : order) {
+        const Acc& a = *ob.second;
+        const bool tab_lds = std::get<0>(ob.first) != 0;
+        if (a.diag > 65536u)
+            return fail(BPX_E_BUDGET, "a scan group needs %u diagonal slots (at most 65536); use smaller query groups",
+                        a.diag);
+        // the bin's largest members together may need fewer waves than each alone
+        const int waves = best_waves(a, tab_lds, std::get<2>(ob.first));
+        if (waves == 0) {
+            const bpx::LdsPlan pl = plan_of(a, tab_lds, std::get<2>(ob.first), any_narrow);
+            r->lds_need = std::max(r->lds_need, pl.total);
+            return fail(BPX_E_BUDGET,
+                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
+                        "use fewer waves per group or a smaller subject buffer",
+                        pl.total, limit, a.diag, std::get<2>(ob.first), sbuf);
+        }
         for (int w = 0; w < 2; ++w) {
             const bool narrow = w == 1;
             const uint32_t lo = narrow ? nwide : 0u, hi = narrow ? count : nwide;
             if (lo >= hi) continue;
-            if (a.diag > 65536u)
-                return fail(BPX_E_BUDGET, "a scan group needs %u diagonal slots (at most 65536); "
-                            "use smaller query groups", a.diag);
+            if ((int)r->classes.size() >= kMaxClasses) return fail(BPX_E_BUDGET, "too many launch classes");
             bpx_result::Class c;
-            c.tab_lds = cls >= 2;
-            c.multi = (cls & 1) != 0;
+            c.tab_lds = tab_lds;
+            c.multi = std::get<1>(ob.first) != 0;
             c.narrow = narrow;
-            int waves = user_waves;
-            bpx::LdsPlan pl{};
-            if (auto_waves) {
-                for (int cand : kAutoWaves) {
-                    waves = cand;
-                    pl = plan_of(a, c.tab_lds, waves, narrow);
-                    if (pl.total <= limit) break;
-                }
-            } else {
-                pl = plan_of(a, c.tab_lds, waves, narrow);
-            }
-            r->lds_need = std::max(r->lds_need, pl.total);
-            if (pl.total > limit)
-                return fail(BPX_E_BUDGET,
-                            "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
-                            "use fewer waves per group or a smaller subject buffer",
-                            pl.total, limit, a.diag, waves, sbuf);
             c.waves = waves;
-            c.plan = pl;
+            c.plan = plan_of(a, tab_lds, waves, narrow);
+            r->lds_need = std::max(r->lds_need, c.plan.total);
             c.diag_cap = std::max<uint32_t>(a.diag, 1);
             c.item_first = (uint32_t)r->items.size();
             for (uint32_t gi : a.groups)
