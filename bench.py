@@ -1,14 +1,20 @@
 """Benchmark of the gfx950 blastp word finder (two-hit word finding +
 ungapped X-drop extension) on one MI355X.
 
-One step = one pass of the hot path over one batch: the 500-residue query
-scanned against the resident 2,000,000-subject database (~719 M residues,
-lognormal lengths median 300 sigma 0.6, 0.5 % planted homologs at 30-70 %
-identity, 0-3 % indels), including the record compaction and the copy of
-records and per-subject counters back to the host.  Inputs are resident in
-HBM before the timed region (database arena and query table uploaded once).
+--workload plain (default): one step = one pass of the hot path over one
+batch: the 500-residue query scanned against the resident 2,000,000-subject
+database (~720 M residues, lognormal lengths median 300 sigma 0.6, 0.5 %
+planted homologs at 30-70 % identity, 0-3 % indels), including the record
+compaction and the copy of records and per-subject counters back to the host.
 
-Prints ONE JSON line (the driver's contract); extra detail goes to stderr.
+--workload batched: 1,024 queries (log-uniform 50-2,000 aa) against a
+500,000-subject database (1 % homologs of random queries at 40-60 %
+identity); one step = one batched scan of all queries (autotuned packing and
+launch shape), compared with scanning the queries one by one.
+
+Inputs are resident in HBM before the timed region (database arena and query
+tables uploaded once).  Prints ONE JSON line (the driver's contract); extra
+detail goes to stderr.
 """
 import argparse
 import json
@@ -35,6 +41,15 @@ def config2(threads):
     spec = dict(seed=2, num_subjects=2_000_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
                 homolog_fraction=0.005, identity_min=0.3, identity_max=0.7, indel_min=0.0, indel_max=0.03)
     return q, bpx.synth_db(spec, [q], nthreads=threads)
+
+
+def config3(threads):
+    import bpx
+    lens = bpx.synth_query_lengths(3, 1024, "loguniform", 50, 2000)
+    qs = [bpx.synth_query(3, i, int(n)) for i, n in enumerate(lens)]
+    spec = dict(seed=3, num_subjects=500_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
+                homolog_fraction=0.01, identity_min=0.4, identity_max=0.6, indel_min=0.0, indel_max=0.03)
+    return qs, bpx.synth_db(spec, qs, nthreads=threads)
 
 
 def cpu_baseline(q, db, threads, budget_s=20.0):
@@ -68,14 +83,19 @@ def cpu_baseline(q, db, threads, budget_s=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; batched 5)")
+    ap.add_argument("--warmup", type=int, default=None, help="warm-up steps (default 3; batched 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--workload", choices=("plain", "batched"), default="plain")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a plain elementwise copy (torch.neg over int32) reading and writing BYTES, "
                          "3 times: the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 5 if args.workload == "batched" else 20
+    if args.warmup is None:
+        args.warmup = 1 if args.workload == "batched" else 3
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder shards by "
               "subject across GPUs, but this build measures the single-GPU path.", file=sys.stderr)
@@ -91,6 +111,9 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {3} x torch.neg over {n} int32 ({4 * n} B read, {4 * n} B written per launch)")
         del a, b
+
+    if args.workload == "batched":
+        return batched(args)
 
     import bpx
     log("generating config-2 workload ...")
@@ -154,6 +177,110 @@ def main():
                       "residues": db.num_residues, "db_sha256": db.sha256,
                       "records": int(nrec), "lookup_hits": ust["lookup_hits"],
                       "init_extends": ust["init_extends"], "device_ms_mean": round(float(np.mean(device_ms)), 4)},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batched(qs, db, threads, budget_s=20.0):
+    """The C oracle over every 16th query of the batch against a bounded
+    prefix of the database, scaled to query-residue pairs per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    sample_q = qs[::16]
+    probe = 500
+    t = time.perf_counter()
+    for q in sample_q:
+        O.run_db(q, db.residues, db.offsets[: probe + 1], db.lengths[:probe], nthreads=threads)
+    dt = max(time.perf_counter() - t, 1e-4)
+    n = int(min(db.num_subjects, max(probe, probe * budget_s / 2 / dt)))
+    t = time.perf_counter()
+    for q in sample_q:
+        O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=threads)
+    dt = time.perf_counter() - t
+    res = int(db.offsets[n]) * len(sample_q)
+    return {"value": res / dt, "unit": "DB residues/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample_q)} of the 1,024 queries (every 16th) against the first {n} subjects; "
+                      f"one timed run after a probe, {threads} threads"}
+
+
+def batched(args):
+    import bpx
+    log("generating the 1,024-query batched workload ...")
+    t0 = time.perf_counter()
+    qs, db = config3(args.threads)
+    qres = int(sum(len(q) for q in qs))
+    log(f"  {len(qs)} queries ({qres} residues), {db.num_subjects} subjects, {db.num_residues} residues, "
+        f"sha256 {db.sha256} ({time.perf_counter() - t0:.1f} s)")
+    ctx = bpx.Context(0)
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    work = db.num_residues * len(qs)  # DB residues summed over queries
+
+    # autotune on a 5 % sample (subjects [0, N/20))
+    t = time.perf_counter()
+    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 20)
+    log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
+        f"{len(trials)} trials)")
+    for tr in trials:
+        log(f"    {tr}")
+    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    info = b.info()
+
+    def step():
+        r = bpx.scan_batch(ctx, b, ddb, launch=launch).wait()
+        t = r.timing()
+        n, _ = r.summary()
+        r.close()
+        return t, n
+
+    for _ in range(args.warmup):
+        step()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tm, nrec = step()
+        kernel_ms.append(tm["kernel_ms"])
+    ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
+    value = work / (ms_per_step / 1000.0)
+
+    # plain: every query on its own (one warm-up pass over 32 queries)
+    dqs = [ctx.prepare_query(q) for q in qs]
+    for dq in dqs[:32]:
+        bpx.scan(ctx, dq, ddb, subject_stats=False).wait().close()
+    t0 = time.perf_counter()
+    plain_records = 0
+    for dq in dqs:
+        r = bpx.scan(ctx, dq, ddb, subject_stats=False).wait()
+        plain_records += r.summary()[0]
+        r.close()
+    plain_s = time.perf_counter() - t0
+    plain_value = work / plain_s
+    if plain_records != nrec:
+        raise SystemExit(f"batched and plain record counts differ: {nrec} vs {plain_records}")
+    log(f"  batched {ms_per_step:.1f} ms/step -> {value / 1e9:.2f} G res/s; plain {plain_s:.2f} s -> "
+        f"{plain_value / 1e9:.2f} G res/s; ratio {value / plain_value:.2f}")
+
+    kern = float(np.mean(kernel_ms))
+    passes = info["groups"]
+    bytes_alg = passes * (db.num_residues + 12 * db.num_subjects) + 12 * db.num_subjects * len(qs) + 24 * nrec
+    achieved = bytes_alg / (kern / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_scan (all launch classes of the batched scan)", "kernel_ms_mean": round(kern, 3),
+                "bytes_alg": int(bytes_alg), "passes": passes}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline_batched(qs, db, args.threads)
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "DB residues/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
+           "data": "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)",
+           "config": {"workload": "batched: 1,024 queries (log-uniform 50-2,000 aa) vs 500,000-subject DB (config 3)",
+                      "queries": len(qs), "query_residues": qres, "subjects": db.num_subjects,
+                      "residues": db.num_residues, "db_sha256": db.sha256, "records": int(nrec),
+                      "groups": passes, "group_residues": gr, "group_queries": gq, "launch": launch,
+                      "plain_value": round(plain_value, 1), "plain_seconds": round(plain_s, 3),
+                      "batched_over_plain": round(value / plain_value, 3)},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

@@ -58,6 +58,14 @@ class BatchPolicy(C.Structure):
     _fields_ = [("group_residues", C.c_int32), ("group_queries", C.c_int32)]
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [("policy", BatchPolicy), ("launch", Launch), ("sample_ms", C.c_float)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("policy", BatchPolicy), ("launch", Launch), ("sample_ms", C.c_float), ("candidates", C.c_int32)]
+
+
 class UngappedStats(C.Structure):
     _fields_ = [("lookup_hits", C.c_int64), ("num_seqs_lookup_hits", C.c_int64),
                 ("init_extends", C.c_int64), ("good_init_extends", C.c_int64),
@@ -118,6 +126,8 @@ _sig("bpx_batch_destroy", C.c_int, _vp)
 _sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
      C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
+_sig("bpx_autotune_batch", C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(WordParams), _vp, _vp, _vp,
+     C.c_uint32, C.c_uint32, C.POINTER(TuneResult), C.POINTER(TuneTrial), C.c_uint32, C.POINTER(C.c_uint32))
 _sig("bpx_blosum62", C.POINTER(C.c_int8))
 _sig("bpx_table_build", C.c_int, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_uint64,
      C.POINTER(C.c_uint64))
@@ -477,3 +487,31 @@ def check_memory(ctx: Context, query: Query, db: Database, first=0, count=None, 
     rc = _lib.bpx_check_memory(ctx._h, db._h, query._h, first, count,
                                C.byref(lp) if lp is not None else None, C.byref(dev), C.byref(lds))
     return rc, int(dev.value), int(lds.value)
+
+
+def _struct_dict(st):
+    return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def autotune_batch(ctx: Context, queries, db: Database, sample_first: int = 0, sample_count: int | None = None,
+                   params: WordParams | None = None, matrix=None, x_dropoff=None, cutoff=None):
+    """Times candidate packing policies / launch shapes on subjects
+    [sample_first, sample_first + sample_count) and returns
+    (group_residues, group_queries, launch dict, trials)."""
+    seqs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+    if sample_count is None:
+        sample_count = max(1, db.num_subjects // 20)
+    ptrs = (C.c_void_p * max(len(seqs), 1))(*[q.ctypes.data for q in seqs])
+    lens = np.array([len(q) for q in seqs] or [0], dtype=np.int32)
+    prm = params if params is not None else default_params()
+    m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.int8)
+    xd = None if x_dropoff is None else np.ascontiguousarray(x_dropoff, dtype=np.int32)
+    co = None if cutoff is None else np.ascontiguousarray(cutoff, dtype=np.int32)
+    out = TuneResult()
+    trials = (TuneTrial * 64)()
+    nt = C.c_uint32()
+    _check(_lib.bpx_autotune_batch(ctx._h, len(seqs), ptrs, _ptr(lens), _ptr(m), C.byref(prm), _ptr(xd), _ptr(co),
+                                   db._h, sample_first, sample_count, C.byref(out), trials, 64, C.byref(nt)))
+    tr = [{"group_residues": t.policy.group_residues, "group_queries": t.policy.group_queries,
+           "launch": _struct_dict(t.launch), "sample_ms": round(float(t.sample_ms), 3)} for t in trials[:min(nt.value, 64)]]
+    return out.policy.group_residues, out.policy.group_queries, _struct_dict(out.launch), tr

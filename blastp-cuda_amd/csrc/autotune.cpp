@@ -1,0 +1,117 @@
+// autotune.cpp -- picks the batch packing policy and the launch shape of a
+// batched scan by timing candidates on a sample of the database.  Written on
+// the public C ABI only (include/bpx.h): every candidate is a real
+// bpx_batch_prepare + bpx_scan_batch, timed by the word-finder kernel time
+// the library measures with HIP events.
+//
+// Search (coordinate descent, each step keeps the best so far):
+//   1. packing policy (group_residues, group_queries) -- how many queries
+//      share one pass over a subject, hence the per-wave diagonal-table
+//      size and the lookup-table placement;
+//   2. subject buffer bytes (per-wave LDS window over the subject);
+//   3. subjects per work item (dynamic scheduling granularity).
+// Results do not depend on any of these, so the tuner only reads timings.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bpx.h"
+
+namespace {
+
+struct Candidate {
+    bpx_batch_policy policy;
+    bpx_launch launch;
+};
+
+// One timed candidate: a warm-up scan (pool allocations, a possible segment
+// rescan), then the faster of two timed scans.
+int time_candidate(bpx_ctx* ctx, const bpx_batch* b, const bpx_db* db, uint32_t first, uint32_t count,
+                   const bpx_launch& launch, float* ms) {
+    float best = 0.f;
+    for (int rep = 0; rep < 3; ++rep) {
+        bpx_result* r = nullptr;
+        int rc = bpx_scan_batch(ctx, b, db, first, count, &launch, 0u, &r);
+        if (rc == BPX_OK) rc = bpx_result_wait(r);
+        float k = 0.f, d = 0.f;
+        if (rc == BPX_OK) rc = bpx_result_timing(r, &k, &d);
+        bpx_result_destroy(r);
+        if (rc != BPX_OK) return rc;
+        if (rep == 1 || (rep == 2 && k < best)) best = k;
+    }
+    *ms = best;
+    return BPX_OK;
+}
+
+}  // namespace
+
+extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint8_t* const* queries,
+                                  const int32_t* qlens, const int8_t* matrix, const bpx_word_params* params,
+                                  const int32_t* x_dropoff, const int32_t* cutoff, const bpx_db* db,
+                                  uint32_t sample_first, uint32_t sample_count, bpx_tune_result* out,
+                                  bpx_tune_trial* trials, uint32_t trial_capacity, uint32_t* num_trials) {
+    if (!ctx || !db || !params || !out) return BPX_E_ARG;
+    uint32_t nsub = 0;
+    int rc = bpx_db_info(db, &nsub, nullptr, nullptr);
+    if (rc) return rc;
+    if ((uint64_t)sample_first + sample_count > nsub || sample_count == 0) return BPX_E_ARG;
+    uint32_t ntried = 0;
+    auto record = [&](const Candidate& c, float ms) {
+        if (trials && ntried < trial_capacity) {
+            trials[ntried].policy = c.policy;
+            trials[ntried].launch = c.launch;
+            trials[ntried].sample_ms = ms;
+        }
+        ++ntried;
+    };
+    auto run = [&](const Candidate& c, float* ms) -> int {
+        bpx_batch* b = nullptr;
+        int r = bpx_batch_prepare(ctx, num_queries, queries, qlens, matrix, params, x_dropoff, cutoff, &c.policy,
+                                  &b);
+        if (r == BPX_OK) r = time_candidate(ctx, b, db, sample_first, sample_count, c.launch, ms);
+        bpx_batch_destroy(b);
+        if (r == BPX_OK) record(c, *ms);
+        return r;
+    };
+
+    Candidate best{};
+    float best_ms = -1.f;
+    auto consider = [&](const Candidate& c) -> int {
+        float ms = 0.f;
+        const int r = run(c, &ms);
+        if (r == BPX_E_BUDGET) return BPX_OK;  // does not fit: not a candidate
+        if (r != BPX_OK) return r;
+        if (best_ms < 0.f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+        return BPX_OK;
+    };
+
+    static const int32_t kPolicies[][2] = {{600, 32}, {1200, 64}, {2400, 128}, {4000, 255}};
+    for (const auto& pq : kPolicies) {
+        Candidate c{};
+        c.policy.group_residues = pq[0];
+        c.policy.group_queries = pq[1];
+        if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    if (best_ms < 0.f) return BPX_E_BUDGET;
+    const Candidate base = best;
+    for (int32_t sb : {1024, 4096}) {
+        Candidate c = base;
+        c.launch.subject_buffer = sb;
+        if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    const Candidate base2 = best;
+    for (int32_t items : {256, 1024}) {
+        Candidate c = base2;
+        c.launch.subjects_per_item = items;
+        if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    out->policy = best.policy;
+    out->launch = best.launch;
+    out->sample_ms = best_ms;
+    out->candidates = (int32_t)ntried;
+    if (num_trials) *num_trials = ntried;
+    return BPX_OK;
+}

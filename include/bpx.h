@@ -208,6 +208,31 @@ int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
                              uint64_t capacity);
 int bpx_result_destroy(bpx_result *r);
 
+/* ---- autotuner ----------------------------------------------------------
+ * Times batched scans of subjects [sample_first, sample_first+sample_count)
+ * under candidate packing policies and launch shapes and returns the
+ * fastest.  No reference counterpart: the reference takes its launch shape
+ * from -gpu_blocks / -gpu_threads (cmdline_flags.cpp:218-222).  Coordinate
+ * search: policy, then subject buffer, then subjects per work item.  The
+ * arguments before db are those of bpx_batch_prepare.  trials (nullable)
+ * receives every configuration timed. */
+typedef struct bpx_tune_trial {
+    bpx_batch_policy policy;
+    bpx_launch launch;
+    float sample_ms;            /* word-finder kernel time on the sample */
+} bpx_tune_trial;
+typedef struct bpx_tune_result {
+    bpx_batch_policy policy;
+    bpx_launch launch;
+    float sample_ms;
+    int32_t candidates;
+} bpx_tune_result;
+int bpx_autotune_batch(bpx_ctx *ctx, uint32_t num_queries, const uint8_t *const *queries,
+                       const int32_t *qlens, const int8_t *matrix, const bpx_word_params *params,
+                       const int32_t *x_dropoff, const int32_t *cutoff, const bpx_db *db,
+                       uint32_t sample_first, uint32_t sample_count, bpx_tune_result *out,
+                       bpx_tune_trial *trials, uint32_t trial_capacity, uint32_t *num_trials);
+
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
 const int8_t *bpx_blosum62(void);
 

@@ -288,11 +288,11 @@ def batch_db(queries, n=3000, seed=31):
     return bpx.synth_db(spec, [q for q in queries if len(q) > 0], nthreads=THREADS)
 
 
-def assert_batch_parity(ctx, queries, db, xdrop=None, cutoff=None, **policy):
+def assert_batch_parity(ctx, queries, db, xdrop=None, cutoff=None, launch=None, **policy):
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
     b = ctx.prepare_batch(queries, x_dropoff=xdrop, cutoff=cutoff, **policy)
     say(f"batch: {b.info()}")
-    r = bpx.scan_batch(ctx, b, ddb, subject_stats=True).wait()
+    r = bpx.scan_batch(ctx, b, ddb, subject_stats=True, launch=launch).wait()
     recs, subs, qids = r.records(with_queries=True)
     st = r.subject_stats().reshape(len(queries), db.num_subjects)
     qstats = r.query_stats()
@@ -328,3 +328,18 @@ def test_batch_per_query_parameters_and_policies(ctx):
                 dict(group_residues=1)):
         say(f"policy {pol}")
         assert_batch_parity(ctx, queries, db, xd, co, **pol)
+
+
+def test_autotuned_batch_matches_oracle(ctx):
+    """The autotuner only reads timings: whatever it picks, the batched scan
+    under that packing and launch shape equals the oracle per query."""
+    rng = np.random.default_rng(23)
+    lens = [int(x) for x in np.exp(rng.uniform(np.log(50), np.log(2000), 24))]
+    queries = [bpx.synth_query(23, i, L) for i, L in enumerate(lens)]
+    db = batch_db(queries, n=2000, seed=33)
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    gr, gq, launch, trials = bpx.autotune_batch(ctx, queries, ddb, 100, 400)
+    say(f"autotune: {gr} {gq} {launch}; {len(trials)} trials")
+    assert len(trials) >= 4 and all(t["sample_ms"] > 0 for t in trials)
+    ddb.close()
+    assert_batch_parity(ctx, queries, db, launch=launch, group_residues=gr, group_queries=gq)
