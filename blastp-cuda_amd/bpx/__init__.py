@@ -32,6 +32,7 @@ ALPHABET = 28
 STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 
 OUT_SUBJECT_STATS = 1
+OUT_GRAPH = 2
 
 HSP_DTYPE = np.dtype([("hsp_q", "<i4"), ("hsp_s", "<i4"), ("query_offset", "<i4"),
                       ("subject_offset", "<i4"), ("hsp_len", "<i4"), ("score", "<i4")])
@@ -126,6 +127,7 @@ _sig("bpx_batch_destroy", C.c_int, _vp)
 _sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
      C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
+_sig("bpx_result_replay", C.c_int, _vp)
 _sig("bpx_autotune_batch", C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(WordParams), _vp, _vp, _vp,
      C.c_uint32, C.c_uint32, C.POINTER(TuneResult), C.POINTER(TuneTrial), C.c_uint32, C.POINTER(C.c_uint32))
 _sig("bpx_blosum62", C.POINTER(C.c_int8))
@@ -441,6 +443,11 @@ class Result:
         _check(_lib.bpx_result_subject_stats(self._h, _ptr(st), len(st)))
         return st[:n]
 
+    def replay(self) -> "Result":
+        """Re-runs a graph-mode scan (same buffers); call .wait() again."""
+        _check(_lib.bpx_result_replay(self._h))
+        return self
+
     def close(self):
         if self._h:
             _lib.bpx_result_destroy(self._h)
@@ -454,11 +461,12 @@ class Result:
 
 
 def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | None = None,
-         launch=None, subject_stats: bool = True) -> Result:
-    """Asynchronous scan of subjects [first, first+count); call .wait()."""
+         launch=None, subject_stats: bool = True, graph: bool = False) -> Result:
+    """Asynchronous scan of subjects [first, first+count); call .wait().
+    graph=True captures the device work in a HIP graph (Result.replay)."""
     if count is None:
         count = db.num_subjects - first
-    flags = OUT_SUBJECT_STATS if subject_stats else 0
+    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0)
     h = _vp()
     lp = _launch(launch)
     _check(_lib.bpx_scan(ctx._h, query._h, db._h, first, count, C.byref(lp) if lp is not None else None,
@@ -467,11 +475,12 @@ def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | 
 
 
 def scan_batch(ctx: Context, batch: Batch, db: Database, first: int = 0, count: int | None = None,
-               launch=None, subject_stats: bool = False) -> Result:
-    """Asynchronous scan of every query of the batch in one persistent launch."""
+               launch=None, subject_stats: bool = False, graph: bool = False) -> Result:
+    """Asynchronous scan of every query of the batch in one persistent launch
+    per launch class; graph=True captures it in a HIP graph (Result.replay)."""
     if count is None:
         count = db.num_subjects - first
-    flags = OUT_SUBJECT_STATS if subject_stats else 0
+    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0)
     h = _vp()
     lp = _launch(launch)
     _check(_lib.bpx_scan_batch(ctx._h, batch._h, db._h, first, count, C.byref(lp) if lp is not None else None,

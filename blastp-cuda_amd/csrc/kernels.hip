@@ -924,25 +924,42 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     return pl;
 }
 
-hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
-                       hipStream_t stream) {
-    if (waves < 1 || waves > kMaxWaves) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)groups), block((unsigned)(64 * waves));
-    void (*fn)(ScanParams) = nullptr;
+namespace {
+using ScanFn = void (*)(ScanParams);
+ScanFn pick_scan(int word, bool table_in_lds, bool multi, bool narrow) {
 #define BPX_PICK(W, N)                                                                                \
-    fn = table_in_lds ? (multi ? k_scan<W, true, true, N> : k_scan<W, true, false, N>)                \
-                      : (multi ? k_scan<W, false, true, N> : k_scan<W, false, false, N>)
-    if (p.word == 3) {
-        if (narrow) BPX_PICK(3, true); else BPX_PICK(3, false);
-    } else if (p.word == 2) {
-        if (narrow) BPX_PICK(2, true); else BPX_PICK(2, false);
-    } else {
-        return hipErrorInvalidValue;
+    return table_in_lds ? (multi ? k_scan<W, true, true, N> : k_scan<W, true, false, N>)              \
+                        : (multi ? k_scan<W, false, true, N> : k_scan<W, false, false, N>)
+    if (word == 3) {
+        if (narrow) BPX_PICK(3, true);
+        BPX_PICK(3, false);
+    }
+    if (word == 2) {
+        if (narrow) BPX_PICK(2, true);
+        BPX_PICK(2, false);
     }
 #undef BPX_PICK
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)p.lds_total);
-    if (e != hipSuccess) return e;
+    return nullptr;
+}
+}  // namespace
+
+hipError_t prepare_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow) {
+    const ScanFn fn = pick_scan(p.word, table_in_lds, multi, narrow);
+    if (!fn) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)p.lds_total);
+}
+
+hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
+                       hipStream_t stream, bool set_attr) {
+    if (waves < 1 || waves > kMaxWaves) return hipErrorInvalidValue;
+    const ScanFn fn = pick_scan(p.word, table_in_lds, multi, narrow);
+    if (!fn) return hipErrorInvalidValue;
+    if (set_attr) {
+        const hipError_t e = prepare_scan(p, table_in_lds, multi, narrow);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((unsigned)groups), block((unsigned)(64 * waves));
     hipLaunchKernelGGL(fn, grid, block, p.lds_total, stream, p);
     return hipGetLastError();
 }

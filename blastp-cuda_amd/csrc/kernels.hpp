@@ -115,8 +115,11 @@ constexpr int kMaxWaves = 16;
 LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, uint32_t diag_cap,
                  uint32_t qcap, uint32_t sbuf, uint32_t claim_slots, int waves, bool table_in_lds, bool narrow);
 
+// Sets the kernel's dynamic-LDS limit for p.lds_total (done by launch_scan
+// unless set_attr is false, e.g. while the stream is being captured).
+hipError_t prepare_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow);
 hipError_t launch_scan(const ScanParams& p, bool table_in_lds, bool multi, bool narrow, int waves, int groups,
-                       hipStream_t stream);
+                       hipStream_t stream, bool set_attr = true);
 
 // Compaction: counts -> offsets (in place) -> records in (query, subject,
 // subject_offset, query_offset) order.

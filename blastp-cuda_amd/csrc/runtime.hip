@@ -207,6 +207,10 @@ struct bpx_result {
     uint64_t total = 0;
     // timing
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // graph mode (BPX_OUT_GRAPH): the captured device work of the scan
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_stale = false;  // buffers were re-allocated by a rescan
     float kernel_ms = 0.f, device_ms = 0.f;
     // launch plan
     struct Class {
@@ -559,19 +563,22 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     return BPX_OK;
 }
 
-int enqueue(bpx_result* r, const uint32_t* order) {
+// The scan's device work on the context stream: counter resets (fill
+// kernels, no memset nodes), every launch class, compaction and the copies of
+// the meta words / subject counters to pinned host memory.  `events` brackets
+// the word-finder launches with the timing events (not when captured into a
+// graph: then the caller brackets the graph launch).  The work items must be
+// on the device already (upload_items).
+int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
     bpx_ctx* ctx = r->ctx;
     const QuerySet& set = *r->set;
     const bpx_db* db = r->db;
     hipStream_t st = ctx->stream;
     HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_meta), 0u, 2 * (kMetaWords + 5ull * set.nq), st));
     HIP_TRY(bpx::launch_fill_u32(ctx->work_counter, 0u, kMaxClasses, st));
-    if (!r->items.empty())
-        HIP_TRY(hipMemcpyAsync(r->d_items, r->items.data(), r->items.size() * sizeof(bpx::WorkItem),
-                               hipMemcpyHostToDevice, st));
     unsigned long long* ustats = r->d_meta + kMetaWords;
     uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
-    HIP_TRY(hipEventRecord(r->ev[0], st));
+    if (events) HIP_TRY(hipEventRecord(r->ev[0], st));
     uint64_t seg_at = 0;
     for (size_t ci = 0; ci < r->classes.size(); ++ci) {
         const auto& c = r->classes[ci];
@@ -622,10 +629,10 @@ int enqueue(bpx_result* r, const uint32_t* order) {
         sp.qcap = c.plan.qcap;
         sp.claim_slots = c.plan.claim_slots;
         sp.sbuf = (uint32_t)r->launch.subject_buffer;
-        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st));
+        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st, events));
         seg_at += (uint64_t)groups * c.waves * r->seg_cap;
     }
-    HIP_TRY(hipEventRecord(r->ev[1], st));
+    if (events) HIP_TRY(hipEventRecord(r->ev[1], st));
     bpx::CompactParams cp{};
     cp.cnt = r->d_cnt;
     cp.n = (uint64_t)set.nq * r->count;
@@ -651,6 +658,64 @@ int enqueue(bpx_result* r, const uint32_t* order) {
         if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
         HIP_TRY(hipMemcpyAsync(r->h_sstats, r->d_stats, pairs * sizeof(bpx_subject_stats), hipMemcpyDeviceToHost, st));
     }
+    return BPX_OK;
+}
+
+int upload_items(bpx_result* r) {
+    if (!r->items.empty())
+        HIP_TRY(hipMemcpyAsync(r->d_items, r->items.data(), r->items.size() * sizeof(bpx::WorkItem),
+                               hipMemcpyHostToDevice, r->ctx->stream));
+    return BPX_OK;
+}
+
+int enqueue(bpx_result* r, const uint32_t* order) {
+    int rc = upload_items(r);
+    return rc ? rc : enqueue_device(r, order, true);
+}
+
+void graph_release(bpx_result* r) {
+    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    r->graph_exec = nullptr;
+    r->graph = nullptr;
+}
+
+// Captures the device work of the scan into a HIP graph and launches it,
+// bracketed by the timing events.
+int graph_capture_and_launch(bpx_result* r) {
+    bpx_ctx* ctx = r->ctx;
+    const uint32_t* order = r->d_order ? r->d_order : r->db->d_order;
+    graph_release(r);
+    int rc = upload_items(r);
+    if (rc) return rc;
+    // everything that allocates or sets kernel attributes happens before the
+    // capture: the pinned counter buffer and each launch's LDS size
+    if ((r->flags & BPX_OUT_SUBJECT_STATS) && r->count && !r->h_sstats) {
+        r->h_sstats = (bpx_subject_stats*)ctx->pinned.get((uint64_t)r->set->nq * r->count * sizeof(bpx_subject_stats));
+        if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
+    }
+    for (const auto& c : r->classes) {
+        bpx::ScanParams sp{};
+        sp.word = r->set->word;
+        sp.lds_total = c.plan.total;
+        HIP_TRY(bpx::prepare_scan(sp, c.tab_lds, c.multi, c.narrow));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_device(r, order, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(BPX_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    r->graph = g;
+    HIP_TRY(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
+    r->graph_stale = false;
+    HIP_TRY(hipEventRecord(r->ev[0], ctx->stream));
+    HIP_TRY(hipGraphLaunch(r->graph_exec, ctx->stream));
+    HIP_TRY(hipEventRecord(r->ev[1], ctx->stream));
     return BPX_OK;
 }
 
@@ -720,7 +785,7 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
         }
         order = r->d_order;
     }
-    rc = enqueue(r, order);
+    rc = (out_flags & BPX_OUT_GRAPH) ? graph_capture_and_launch(r) : enqueue(r, order);
     if (rc) {
         bpx_result_destroy(r);
         return rc;
@@ -1022,6 +1087,7 @@ int bpx_result_wait(bpx_result* r) {
         if (rc) return rc;
         rc = enqueue(r, order);
         if (rc) return rc;
+        r->graph_stale = r->graph != nullptr;
         ++r->reruns;
     }
     r->total = r->h_meta[0];
@@ -1110,10 +1176,34 @@ int bpx_result_subject_stats(const bpx_result* r, bpx_subject_stats* stats, uint
     return BPX_OK;
 }
 
+int bpx_result_replay(bpx_result* r) {
+    if (!r) return fail(BPX_E_ARG, "result is NULL");
+    if (!(r->flags & BPX_OUT_GRAPH)) return fail(BPX_E_STATE, "not a graph-mode result (scan without BPX_OUT_GRAPH)");
+    if (!r->waited) return fail(BPX_E_STATE, "replay before the previous run was waited");
+    bpx_ctx* ctx = r->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    // the previous run's host copies of the records go back to the pool;
+    // bpx_result_wait fetches the new ones
+    ctx->pinned.put(r->h_records);
+    ctx->pinned.put(r->h_subjects);
+    ctx->pinned.put(r->h_queries);
+    r->h_records = nullptr;
+    r->h_subjects = nullptr;
+    r->h_queries = nullptr;
+    r->waited = false;
+    r->total = 0;
+    if (r->graph_stale || !r->graph_exec) return graph_capture_and_launch(r);
+    HIP_TRY(hipEventRecord(r->ev[0], ctx->stream));
+    HIP_TRY(hipGraphLaunch(r->graph_exec, ctx->stream));
+    HIP_TRY(hipEventRecord(r->ev[1], ctx->stream));
+    return BPX_OK;
+}
+
 int bpx_result_destroy(bpx_result* r) {
     if (!r) return BPX_OK;
     (void)hipSetDevice(r->ctx->device);
     (void)hipStreamSynchronize(r->ctx->stream);
+    graph_release(r);
     result_release(r, false);
     for (auto& ev : r->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -122,6 +122,7 @@ typedef struct bpx_batch_policy {
 
 /* Output selection for a scan. */
 #define BPX_OUT_SUBJECT_STATS 1u /* copy per-subject counters to the host */
+#define BPX_OUT_GRAPH 2u         /* capture the scan's device work in a HIP graph (bpx_result_replay) */
 
 /* ---- device / context ------------------------------------------------- */
 int bpx_available(void);
@@ -206,6 +207,13 @@ int bpx_result_records(const bpx_result *r, bpx_init_hsp *records, uint32_t *sub
  * available when the scan was made with BPX_OUT_SUBJECT_STATS. */
 int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
                              uint64_t capacity);
+/* Graph mode (scan made with BPX_OUT_GRAPH): re-runs the captured device
+ * work of the scan -- counter resets, every launch class, compaction, copies
+ * -- as one hipGraphLaunch on the same buffers, for the same queries,
+ * database and range.  Call after bpx_result_wait; wait again before reading.
+ * Timing then covers the whole graph.  No reference counterpart: the
+ * reference re-issues every copy and launch per search (gpu_blastp.cu:139-201). */
+int bpx_result_replay(bpx_result *r);
 int bpx_result_destroy(bpx_result *r);
 
 /* ---- autotuner ----------------------------------------------------------

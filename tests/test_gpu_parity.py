@@ -343,3 +343,37 @@ def test_autotuned_batch_matches_oracle(ctx):
     assert len(trials) >= 4 and all(t["sample_ms"] > 0 for t in trials)
     ddb.close()
     assert_batch_parity(ctx, queries, db, launch=launch, group_residues=gr, group_queries=gq)
+
+
+def test_graph_replay_equals_eager(ctx):
+    """A graph-mode batched scan replayed several times, with eager scans in
+    between, gives the eager result on every replay (records, subject and
+    query ids, counters, per-query stats)."""
+    rng = np.random.default_rng(24)
+    lens = [int(x) for x in rng.integers(20, 700, 30)]
+    queries = [bpx.synth_query(24, i, L) for i, L in enumerate(lens)]
+    db = batch_db(queries, n=2500, seed=34)
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    b = ctx.prepare_batch(queries, group_residues=900, group_queries=16)
+
+    def snapshot(r):
+        recs, subs, qids = r.records(with_queries=True)
+        return (recs.view(np.int32).copy(), subs.copy(), qids.copy(), r.subject_stats().view(np.int32).copy(),
+                [tuple(d.values()) for d in r.query_stats()])
+
+    eager = bpx.scan_batch(ctx, b, ddb, subject_stats=True).wait()
+    want = snapshot(eager)
+    eager.close()
+    g = bpx.scan_batch(ctx, b, ddb, subject_stats=True, graph=True).wait()
+    for rep in range(4):
+        got = snapshot(g)
+        for x, y in zip(got[:4], want[:4]):
+            np.testing.assert_array_equal(x, y)
+        assert got[4] == want[4], rep
+        # eager work and a synchronisation between replays
+        other = bpx.scan_batch(ctx, b, ddb, first=100, count=500).wait()
+        other.close()
+        g.replay().wait()
+    g.close()
+    b.close()
+    ddb.close()
