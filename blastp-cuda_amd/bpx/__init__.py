@@ -128,6 +128,15 @@ _sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER
      C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
 _sig("bpx_result_replay", C.c_int, _vp)
+_sig("bpx_db_features", C.c_int, _vp, _vp, _vp, C.c_int32, _vp)
+_sig("bpx_db_set_cost", C.c_int, _vp, _vp)
+_sig("bpx_costmodel_create", C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp))
+_sig("bpx_costmodel_params", C.c_int, _vp, _vp, C.c_int)
+_sig("bpx_costmodel_train", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _vp)
+_sig("bpx_costmodel_predict", C.c_int, _vp, _vp, C.c_uint32, _vp)
+_sig("bpx_costmodel_grad", C.c_int, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_float), _vp)
+_sig("bpx_costmodel_last_error", C.c_char_p)
+_sig("bpx_costmodel_destroy", C.c_int, _vp)
 _sig("bpx_autotune_batch", C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, C.POINTER(WordParams), _vp, _vp, _vp,
      C.c_uint32, C.c_uint32, C.POINTER(TuneResult), C.POINTER(TuneTrial), C.c_uint32, C.POINTER(C.c_uint32))
 _sig("bpx_blosum62", C.POINTER(C.c_int8))
@@ -524,3 +533,91 @@ def autotune_batch(ctx: Context, queries, db: Database, sample_first: int = 0, s
     tr = [{"group_residues": t.policy.group_residues, "group_queries": t.policy.group_queries,
            "launch": _struct_dict(t.launch), "sample_ms": round(float(t.sample_ms), 3)} for t in trials[:min(nt.value, 64)]]
     return out.policy.group_residues, out.policy.group_queries, _struct_dict(out.launch), tr
+
+
+CM_FEATURES = 32
+CM_PARAMS = 7312
+
+
+def word_counts(queries, threshold=11, word_length=3, matrix=None) -> np.ndarray:
+    """Sum over the queries of each word's lookup-cell size (the number of
+    hits a subject word would make against all of them), saturated to u16."""
+    tot = np.zeros(1 << (5 * word_length), dtype=np.int64)
+    for q in queries:
+        cs, _ = table_build(q, threshold, word_length, matrix)
+        tot += np.diff(cs.astype(np.int64))
+    return np.minimum(tot, 65535).astype(np.uint16)
+
+
+def db_features(ctx: Context, db: Database, counts: np.ndarray, word_length=3) -> np.ndarray:
+    out = np.zeros((max(db.num_subjects, 1), CM_FEATURES), dtype=np.float32)
+    wc = np.ascontiguousarray(counts, dtype=np.uint16)
+    _check(_lib.bpx_db_features(ctx._h, db._h, _ptr(wc), word_length, _ptr(out)))
+    return out[:db.num_subjects]
+
+
+def db_set_cost(db: Database, cost=None):
+    """Order (and bin) the database's scan work by descending cost; None
+    restores the longest-first default."""
+    c = None if cost is None else np.ascontiguousarray(cost, dtype=np.float32)
+    _check(_lib.bpx_db_set_cost(db._h, _ptr(c)))
+
+
+class CostModel:
+    """32-64-64-1 MLP (bf16 weights/activations, fp32 accumulation) trained
+    and run on the GPU by the library's MFMA kernels."""
+
+    def __init__(self, device=0, seed=0):
+        self._h = _vp()
+        rc = _lib.bpx_costmodel_create(device, seed, C.byref(self._h))
+        if rc:
+            raise BpxError(rc, _lib.bpx_costmodel_last_error().decode())
+
+    def _chk(self, rc):
+        if rc:
+            raise BpxError(rc, _lib.bpx_costmodel_last_error().decode())
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(CM_PARAMS, dtype=np.float32)
+        self._chk(_lib.bpx_costmodel_params(self._h, _ptr(p), 0))
+        return p
+
+    @params.setter
+    def params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.shape == (CM_PARAMS,)
+        self._chk(_lib.bpx_costmodel_params(self._h, _ptr(a), 1))
+
+    def train(self, X, y, epochs=8, batch=4096, lr=3e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._chk(_lib.bpx_costmodel_train(self._h, _ptr(X), _ptr(y), len(y), epochs, batch, lr, seed,
+                                           _ptr(losses)))
+        return losses
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(max(len(X), 1), dtype=np.float32)
+        self._chk(_lib.bpx_costmodel_predict(self._h, _ptr(X), len(X), _ptr(out)))
+        return out[:len(X)]
+
+    def grad(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(CM_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._chk(_lib.bpx_costmodel_grad(self._h, _ptr(X), _ptr(y), len(y), C.byref(loss), _ptr(g)))
+        return float(loss.value), g
+
+    def close(self):
+        if self._h:
+            _lib.bpx_costmodel_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -3,6 +3,7 @@
 // their results.  One HIP stream per context; every device buffer is owned
 // by a handle.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,8 @@
 
 namespace bpx {
 const int8_t* blosum62();
+hipError_t launch_features(const uint8_t* residues, const uint64_t* soff, const uint32_t* slen, uint32_t n,
+                           const uint16_t* word_counts, int word, float* out, int num_cus, hipStream_t stream);
 }
 
 namespace {
@@ -130,7 +133,12 @@ struct bpx_db {
     uint32_t* d_order = nullptr;  // all subjects, longest first
     std::vector<uint32_t> h_len;
     std::vector<uint32_t> h_order;
+    // set by bpx_db_set_cost: prefix sums of the cost over the processing
+    // order (num_subjects + 1), for cost-guided work items
+    std::vector<double> cost_prefix;
 };
+constexpr uint32_t kCostLongPrefix = 15360;  // cost orders keep subjects this long first, longest first
+constexpr int32_t kCostMaxWindow = bpx::kNarrowLimit - kCostLongPrefix;
 
 // Queries staged on the device as scan groups.
 struct QuerySet {
@@ -349,7 +357,8 @@ bpx_launch resolve_launch(const bpx_launch* in) {
 // Splits the set's groups into launch classes and lays out LDS for each.
 // Subjects [0, nwide) of the processing order (the longest, length + window
 // >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
-int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
+int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user,
+              const std::vector<double>* cost_prefix = nullptr) {
     bpx_ctx* ctx = r->ctx;
     const bpx_launch L = resolve_launch(user);
     const bool auto_waves = !user || user->waves_per_group <= 0;
@@ -475,8 +484,27 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             r->lds_need = std::max(r->lds_need, c.plan.total);
             c.diag_cap = std::max<uint32_t>(a.diag, 1);
             c.item_first = (uint32_t)r->items.size();
-            for (uint32_t gi : a.groups)
-                for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
+            if (cost_prefix && L.subjects_per_item <= 0) {
+                // guided by cost: each item takes a share of the cost left
+                // in the range, so items shrink toward the end of the queue
+                const std::vector<double>& P = *cost_prefix;
+                const double nwg = (double)std::max(1, ctx->num_cus * L.groups_per_cu);
+                const double floor_cost = (P[hi] - P[lo]) / (nwg * 64.0);
+                std::vector<std::pair<uint32_t, uint32_t>> cuts;
+                for (uint32_t f = lo; f < hi;) {
+                    const double target = std::max((P[hi] - P[f]) / (2.0 * nwg), floor_cost);
+                    uint32_t e = (uint32_t)(std::lower_bound(P.begin() + f + 1, P.begin() + hi + 1, P[f] + target) -
+                                            P.begin());
+                    e = std::min(std::max(e, f + 1), std::min(hi, f + 4096u));
+                    cuts.push_back({f, e - f});
+                    f = e;
+                }
+                for (uint32_t gi : a.groups)
+                    for (const auto& ct : cuts) r->items.push_back({gi, ct.first, ct.second, 0});
+            } else {
+                for (uint32_t gi : a.groups)
+                    for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
+            }
             c.nitems = (uint32_t)r->items.size() - c.item_first;
             r->classes.push_back(c);
         }
@@ -745,7 +773,13 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     r->first = first;
     r->count = count;
     r->flags = out_flags;
-    int rc = plan_scan(r, set, count, count_wide(db, first, count, set.window), launch);
+    const bool full = first == 0 && count == db->n;
+    if (!db->cost_prefix.empty() && set.window > kCostMaxWindow) {
+        delete r;
+        return fail(BPX_E_ARG, "window %d above %d while the database has a cost order", set.window, kCostMaxWindow);
+    }
+    int rc = plan_scan(r, set, count, count_wide(db, first, count, set.window), launch,
+                       full && !db->cost_prefix.empty() ? &db->cost_prefix : nullptr);
     if (rc) {
         delete r;
         return rc;
@@ -908,6 +942,63 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
         return fail(BPX_E_HIP, "arena upload failed: %s", hipGetErrorString(e));
     }
     *out = db;
+    return BPX_OK;
+}
+
+int bpx_db_set_cost(bpx_db* db, const float* cost) {
+    if (!db) return fail(BPX_E_ARG, "db is NULL");
+    HIP_TRY(hipSetDevice(db->ctx->device));
+    std::vector<uint32_t> order(db->n);
+    std::iota(order.begin(), order.end(), 0u);
+    if (cost) {
+        for (uint32_t i = 0; i < db->n; ++i)
+            if (!(cost[i] >= 0.f) || !std::isfinite(cost[i]))
+                return fail(BPX_E_ARG, "cost[%u] is negative or not finite", i);
+        // long subjects (wide diagonal slots) first, longest first; then by cost
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            const bool la = db->h_len[a] >= kCostLongPrefix, lb = db->h_len[b] >= kCostLongPrefix;
+            if (la != lb) return la;
+            if (la) return db->h_len[a] > db->h_len[b];
+            return cost[a] > cost[b];
+        });
+    } else {
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
+    }
+    if (db->n) HIP_TRY(hipMemcpy(db->d_order, order.data(), (size_t)db->n * 4, hipMemcpyHostToDevice));
+    db->h_order.swap(order);
+    db->cost_prefix.clear();
+    if (cost) {
+        db->cost_prefix.resize((size_t)db->n + 1, 0.0);
+        for (uint32_t k = 0; k < db->n; ++k) db->cost_prefix[k + 1] = db->cost_prefix[k] + cost[db->h_order[k]];
+    }
+    return BPX_OK;
+}
+
+int bpx_db_features(bpx_ctx* ctx, const bpx_db* db, const uint16_t* word_counts, int32_t word_length,
+                    float* features) {
+    if (!ctx || !db || !word_counts || !features) return fail(BPX_E_ARG, "NULL argument");
+    if (word_length != 2 && word_length != 3) return fail(BPX_E_ARG, "word_length must be 2 or 3");
+    if (db->ctx != ctx) return fail(BPX_E_ARG, "database belongs to another context");
+    if (db->n == 0) return BPX_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nw = (size_t)1 << (5 * word_length);
+    uint16_t* d_counts = (uint16_t*)ctx->pool.get(nw * 2);
+    float* d_out = (float*)ctx->pool.get((size_t)db->n * BPX_CM_FEATURES * 4);
+    if (!d_counts || !d_out) {
+        ctx->pool.put(d_counts);
+        ctx->pool.put(d_out);
+        return fail(BPX_E_NOMEM, "feature buffers");
+    }
+    hipError_t e = hipMemcpyAsync(d_counts, word_counts, nw * 2, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = bpx::launch_features(db->d_arena, db->d_off, db->d_len, db->n, d_counts, word_length, d_out, ctx->num_cus,
+                                 ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(features, d_out, (size_t)db->n * BPX_CM_FEATURES * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->pool.put(d_counts);
+    ctx->pool.put(d_out);
+    if (e != hipSuccess) return fail(BPX_E_HIP, "features: %s", hipGetErrorString(e));
     return BPX_OK;
 }
 

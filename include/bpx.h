@@ -241,6 +241,42 @@ int bpx_autotune_batch(bpx_ctx *ctx, uint32_t num_queries, const uint8_t *const 
                        uint32_t sample_first, uint32_t sample_count, bpx_tune_result *out,
                        bpx_tune_trial *trials, uint32_t trial_capacity, uint32_t *num_trials);
 
+/* ---- learned cost model --------------------------------------------------
+ * No reference counterpart (the reference orders subjects by length only,
+ * functions.cpp:125-149).  A 32-64-64-1 ReLU MLP with bf16 weights and
+ * activations and fp32 accumulation, trained and run on the GPU by the
+ * library's own MFMA kernels.  It predicts per-subject scan work from
+ * per-subject features and is used only to order and bin work
+ * (bpx_db_set_cost); results never depend on it. */
+#define BPX_CM_FEATURES 32  /* feature vector width (22 used, rest zero) */
+#define BPX_CM_PARAMS 7312  /* W1[32][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16] */
+typedef struct bpx_costmodel bpx_costmodel;
+/* Features of every subject, computed on the GPU, [num_subjects x 32] fp32
+ * row-major: log(1 + length), the composition fractions of the 20 standard
+ * residues (ARNDCQEGHILKMFPSTWYV), log(1 + sum over the subject's words of
+ * word_counts[word]) (a sampled query-word hit count), zeros. */
+int bpx_db_features(bpx_ctx *ctx, const bpx_db *db, const uint16_t *word_counts,
+                    int32_t word_length, float *features);
+int bpx_costmodel_create(int device, uint64_t seed, bpx_costmodel **out);
+/* get (set = 0) or set the fp32 parameters (BPX_CM_PARAMS floats) */
+int bpx_costmodel_params(bpx_costmodel *cm, float *params, int set);
+/* Adam on mean squared error; epoch_loss (nullable) gets each epoch's mean loss. */
+int bpx_costmodel_train(bpx_costmodel *cm, const float *features, const float *target,
+                        uint32_t n, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                        float *epoch_loss);
+int bpx_costmodel_predict(bpx_costmodel *cm, const float *features, uint32_t n, float *out);
+/* Mean squared error and its gradient (BPX_CM_PARAMS floats) over n rows. */
+int bpx_costmodel_grad(bpx_costmodel *cm, const float *features, const float *target,
+                       uint32_t n, float *loss, float *grad);
+const char *bpx_costmodel_last_error(void);
+int bpx_costmodel_destroy(bpx_costmodel *cm);
+/* Processing order and work items by predicted cost: subjects of length >=
+ * 15,360 first (longest first), then the rest by descending cost, cut into
+ * work items of decreasing cost (guided: each takes a share of what remains).
+ * cost = NULL restores the default longest-first order with equal-count
+ * items.  Scans with a window above 1,024 are refused while costs are set. */
+int bpx_db_set_cost(bpx_db *db, const float *cost);
+
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
 const int8_t *bpx_blosum62(void);
 
