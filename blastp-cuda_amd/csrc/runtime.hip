@@ -133,9 +133,7 @@ struct bpx_db {
     uint32_t* d_order = nullptr;  // all subjects, longest first
     std::vector<uint32_t> h_len;
     std::vector<uint32_t> h_order;
-    // set by bpx_db_set_cost: prefix sums of the cost over the processing
-    // order (num_subjects + 1), for cost-guided work items
-    std::vector<double> cost_prefix;
+    bool cost_order = false;  // h_order set by bpx_db_set_cost
 };
 constexpr uint32_t kCostLongPrefix = 15360;  // cost orders keep subjects this long first, longest first
 constexpr int32_t kCostMaxWindow = bpx::kNarrowLimit - kCostLongPrefix;
@@ -357,8 +355,7 @@ bpx_launch resolve_launch(const bpx_launch* in) {
 // Splits the set's groups into launch classes and lays out LDS for each.
 // Subjects [0, nwide) of the processing order (the longest, length + window
 // >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
-int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user,
-              const std::vector<double>* cost_prefix = nullptr) {
+int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
     bpx_ctx* ctx = r->ctx;
     const bpx_launch L = resolve_launch(user);
     const bool auto_waves = !user || user->waves_per_group <= 0;
@@ -484,27 +481,8 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             r->lds_need = std::max(r->lds_need, c.plan.total);
             c.diag_cap = std::max<uint32_t>(a.diag, 1);
             c.item_first = (uint32_t)r->items.size();
-            if (cost_prefix && L.subjects_per_item <= 0) {
-                // guided by cost: each item takes a share of the cost left
-                // in the range, so items shrink toward the end of the queue
-                const std::vector<double>& P = *cost_prefix;
-                const double nwg = (double)std::max(1, ctx->num_cus * L.groups_per_cu);
-                const double floor_cost = (P[hi] - P[lo]) / (nwg * 64.0);
-                std::vector<std::pair<uint32_t, uint32_t>> cuts;
-                for (uint32_t f = lo; f < hi;) {
-                    const double target = std::max((P[hi] - P[f]) / (2.0 * nwg), floor_cost);
-                    uint32_t e = (uint32_t)(std::lower_bound(P.begin() + f + 1, P.begin() + hi + 1, P[f] + target) -
-                                            P.begin());
-                    e = std::min(std::max(e, f + 1), std::min(hi, f + 4096u));
-                    cuts.push_back({f, e - f});
-                    f = e;
-                }
-                for (uint32_t gi : a.groups)
-                    for (const auto& ct : cuts) r->items.push_back({gi, ct.first, ct.second, 0});
-            } else {
-                for (uint32_t gi : a.groups)
-                    for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
-            }
+            for (uint32_t gi : a.groups)
+                for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
             c.nitems = (uint32_t)r->items.size() - c.item_first;
             r->classes.push_back(c);
         }
@@ -773,13 +751,11 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     r->first = first;
     r->count = count;
     r->flags = out_flags;
-    const bool full = first == 0 && count == db->n;
-    if (!db->cost_prefix.empty() && set.window > kCostMaxWindow) {
+    if (db->cost_order && set.window > kCostMaxWindow) {
         delete r;
         return fail(BPX_E_ARG, "window %d above %d while the database has a cost order", set.window, kCostMaxWindow);
     }
-    int rc = plan_scan(r, set, count, count_wide(db, first, count, set.window), launch,
-                       full && !db->cost_prefix.empty() ? &db->cost_prefix : nullptr);
+    int rc = plan_scan(r, set, count, count_wide(db, first, count, set.window), launch);
     if (rc) {
         delete r;
         return rc;
@@ -966,11 +942,7 @@ int bpx_db_set_cost(bpx_db* db, const float* cost) {
     }
     if (db->n) HIP_TRY(hipMemcpy(db->d_order, order.data(), (size_t)db->n * 4, hipMemcpyHostToDevice));
     db->h_order.swap(order);
-    db->cost_prefix.clear();
-    if (cost) {
-        db->cost_prefix.resize((size_t)db->n + 1, 0.0);
-        for (uint32_t k = 0; k < db->n; ++k) db->cost_prefix[k + 1] = db->cost_prefix[k] + cost[db->h_order[k]];
-    }
+    db->cost_order = cost != nullptr;
     return BPX_OK;
 }
 

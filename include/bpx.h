@@ -270,11 +270,11 @@ int bpx_costmodel_grad(bpx_costmodel *cm, const float *features, const float *ta
                        uint32_t n, float *loss, float *grad);
 const char *bpx_costmodel_last_error(void);
 int bpx_costmodel_destroy(bpx_costmodel *cm);
-/* Processing order and work items by predicted cost: subjects of length >=
- * 15,360 first (longest first), then the rest by descending cost, cut into
- * work items of decreasing cost (guided: each takes a share of what remains).
- * cost = NULL restores the default longest-first order with equal-count
- * items.  Scans with a window above 1,024 are refused while costs are set. */
+/* Processing order by predicted cost: subjects of length >= 15,360 first
+ * (longest first), then the rest by descending cost; work items stay
+ * equal-count ranges of that order.  cost = NULL restores the default
+ * longest-first order.  Scans with a window above 1,024 are refused while a
+ * cost order is set.  Not while a scan of this database is in flight. */
 int bpx_db_set_cost(bpx_db *db, const float *cost);
 
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
