@@ -59,6 +59,12 @@ class BatchPolicy(C.Structure):
     _fields_ = [("group_residues", C.c_int32), ("group_queries", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("max_inflight", C.c_int32), ("max_wait_us", C.c_int32),
+                ("reserved", C.c_int32), ("hbm_budget", C.c_int64), ("params", WordParams),
+                ("policy", BatchPolicy), ("launch", Launch)]
+
+
 class TuneTrial(C.Structure):
     _fields_ = [("policy", BatchPolicy), ("launch", Launch), ("sample_ms", C.c_float)]
 
@@ -74,6 +80,11 @@ class UngappedStats(C.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("num_records", C.c_uint64), ("stats", UngappedStats), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("batch_size", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class SynthSpec(C.Structure):
@@ -128,6 +139,13 @@ _sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER
      C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
 _sig("bpx_result_replay", C.c_int, _vp)
+_sig("bpx_result_ready", C.c_int, _vp)
+_sig("bpx_server_create", C.c_int, _vp, _vp, C.POINTER(ServerConfig), C.POINTER(_vp))
+_sig("bpx_server_submit", C.c_int, _vp, _vp, C.c_int32, C.POINTER(C.c_uint64))
+_sig("bpx_server_wait", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(RequestInfo))
+_sig("bpx_server_records", C.c_int, _vp, C.c_uint64, _vp, _vp, C.c_uint64)
+_sig("bpx_server_counters", C.c_int, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+_sig("bpx_server_destroy", C.c_int, _vp)
 _sig("bpx_db_features", C.c_int, _vp, _vp, _vp, C.c_int32, _vp)
 _sig("bpx_db_set_cost", C.c_int, _vp, _vp)
 _sig("bpx_costmodel_create", C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp))
@@ -614,6 +632,60 @@ class CostModel:
     def close(self):
         if self._h:
             _lib.bpx_costmodel_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching query server over a resident database (include/bpx.h
+    bpx_server_*): submit() queues a query, wait() blocks for it, records()
+    fetches its records.  Each result equals scanning the query alone."""
+
+    def __init__(self, ctx: Context, db: Database, max_batch=0, max_inflight=0, max_wait_us=0, hbm_budget=0,
+                 group_residues=0, group_queries=0, launch=None, params: WordParams | None = None):
+        cfg = ServerConfig()
+        cfg.max_batch, cfg.max_inflight, cfg.max_wait_us, cfg.hbm_budget = max_batch, max_inflight, max_wait_us, hbm_budget
+        if params is not None:
+            cfg.params = params
+        cfg.policy = BatchPolicy(group_residues, group_queries)
+        if launch is not None:
+            cfg.launch = _launch(launch)
+        self.ctx, self.db = ctx, db
+        self._h = _vp()
+        _check(_lib.bpx_server_create(ctx._h, db._h, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, query) -> int:
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        t = C.c_uint64()
+        _check(_lib.bpx_server_submit(self._h, _ptr(q), len(q), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> dict:
+        info = RequestInfo()
+        _check(_lib.bpx_server_wait(self._h, ticket, timeout_ms, C.byref(info)))
+        return {"num_records": int(info.num_records), "stats": info.stats.as_dict(),
+                "latency_ms": float(info.latency_ms), "queue_ms": float(info.queue_ms),
+                "batch_size": int(info.batch_size)}
+
+    def records(self, ticket: int, n: int):
+        rec = np.zeros(max(n, 1), dtype=HSP_DTYPE)
+        sub = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(_lib.bpx_server_records(self._h, ticket, _ptr(rec), _ptr(sub), len(rec)))
+        return rec[:n], sub[:n]
+
+    def counters(self):
+        b, c = C.c_uint64(), C.c_uint64()
+        _check(_lib.bpx_server_counters(self._h, C.byref(b), C.byref(c)))
+        return {"batches": b.value, "completed": c.value}
+
+    def close(self):
+        if self._h:
+            _lib.bpx_server_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

@@ -116,7 +116,8 @@ struct bpx_ctx {
     int num_cus = 0;
     int64_t lds_limit = 0;
     int64_t hbm_bytes = 0;
-    uint32_t* work_counter = nullptr;  // one per launch class
+    hipStream_t lanes[2] = {nullptr, nullptr};  // scans alternate between two streams, so
+    uint32_t next_lane = 0;                     // consecutive scans may overlap on the device
     uint32_t seg_hint = 0;             // per-wave record segment size a previous scan needed
     Pool pool;
     Pool pinned{true};
@@ -212,6 +213,8 @@ struct bpx_result {
     bpx_subject_stats* h_sstats = nullptr;
     uint64_t total = 0;
     // timing
+    hipStream_t stream = nullptr;    // the context lane this scan runs on
+    uint32_t* d_work = nullptr;      // work counters, one per launch class
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     // graph mode (BPX_OUT_GRAPH): the captured device work of the scan
     hipGraph_t graph = nullptr;
@@ -507,6 +510,8 @@ void result_release(bpx_result* r, bool keep_order) {
     pool.put(r->d_queries);
     pool.put(r->d_meta);
     pool.put(r->d_items);
+    pool.put(r->d_work);
+    r->d_work = nullptr;
     if (!keep_order) {
         pool.put(r->d_order);
         r->d_order = nullptr;
@@ -560,9 +565,11 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     r->d_queries = (uint32_t*)pool.get(r->seg_total * 4);
     r->d_meta = (unsigned long long*)pool.get((kMetaWords + 5ull * set.nq) * 8);
     r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
+    r->d_work = (uint32_t*)pool.get(kMaxClasses * 4);
     r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
     if (!r->d_cnt || ((r->flags & BPX_OUT_SUBJECT_STATS) && !r->d_stats) || !r->d_run_loc || !r->d_run_len ||
         !r->d_seg || !r->d_tile || !r->d_records || !r->d_subjects || !r->d_queries || !r->d_meta || !r->d_items ||
+        !r->d_work ||
         !r->h_meta)
         return fail(BPX_E_NOMEM, "allocation for scan results failed (%llu query-subject pairs, %llu segment records)",
                     (unsigned long long)pairs, (unsigned long long)r->seg_total);
@@ -579,9 +586,9 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
     bpx_ctx* ctx = r->ctx;
     const QuerySet& set = *r->set;
     const bpx_db* db = r->db;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = r->stream;
     HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_meta), 0u, 2 * (kMetaWords + 5ull * set.nq), st));
-    HIP_TRY(bpx::launch_fill_u32(ctx->work_counter, 0u, kMaxClasses, st));
+    HIP_TRY(bpx::launch_fill_u32(r->d_work, 0u, kMaxClasses, st));
     unsigned long long* ustats = r->d_meta + kMetaWords;
     uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
     if (events) HIP_TRY(hipEventRecord(r->ev[0], st));
@@ -606,7 +613,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
         sp.window = set.window;
         sp.items = r->d_items + c.item_first;
         sp.nitems = c.nitems;
-        sp.work_counter = ctx->work_counter + ci;
+        sp.work_counter = r->d_work + ci;
         sp.cnt = r->d_cnt;
         sp.stats = r->d_stats;
         sp.run_loc = r->d_run_loc;
@@ -670,7 +677,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
 int upload_items(bpx_result* r) {
     if (!r->items.empty())
         HIP_TRY(hipMemcpyAsync(r->d_items, r->items.data(), r->items.size() * sizeof(bpx::WorkItem),
-                               hipMemcpyHostToDevice, r->ctx->stream));
+                               hipMemcpyHostToDevice, r->stream));
     return BPX_OK;
 }
 
@@ -706,11 +713,11 @@ int graph_capture_and_launch(bpx_result* r) {
         sp.lds_total = c.plan.total;
         HIP_TRY(bpx::prepare_scan(sp, c.tab_lds, c.multi, c.narrow));
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
     rc = enqueue_device(r, order, false);
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    const hipError_t e = hipStreamEndCapture(r->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -719,9 +726,9 @@ int graph_capture_and_launch(bpx_result* r) {
     r->graph = g;
     HIP_TRY(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
     r->graph_stale = false;
-    HIP_TRY(hipEventRecord(r->ev[0], ctx->stream));
-    HIP_TRY(hipGraphLaunch(r->graph_exec, ctx->stream));
-    HIP_TRY(hipEventRecord(r->ev[1], ctx->stream));
+    HIP_TRY(hipEventRecord(r->ev[0], r->stream));
+    HIP_TRY(hipGraphLaunch(r->graph_exec, r->stream));
+    HIP_TRY(hipEventRecord(r->ev[1], r->stream));
     return BPX_OK;
 }
 
@@ -751,6 +758,7 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     r->first = first;
     r->count = count;
     r->flags = out_flags;
+    r->stream = ctx->lanes[ctx->next_lane++ & 1u];
     if (db->cost_order && set.window > kCostMaxWindow) {
         delete r;
         return fail(BPX_E_ARG, "window %d above %d while the database has a cost order", set.window, kCostMaxWindow);
@@ -786,8 +794,8 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
             return fail(BPX_E_NOMEM, "work list allocation failed");
         }
         if (count) {
-            hipError_t e = hipMemcpyAsync(r->d_order, sub.data(), (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            hipError_t e = hipMemcpyAsync(r->d_order, sub.data(), (size_t)count * 4, hipMemcpyHostToDevice, r->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
             if (e != hipSuccess) {
                 bpx_result_destroy(r);
                 return fail(BPX_E_HIP, "work list upload failed: %s", hipGetErrorString(e));
@@ -836,7 +844,8 @@ int bpx_context_create(int device, bpx_ctx** out) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) ctx->hbm_bytes = (int64_t)tot;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->work_counter, 256) != hipSuccess) {
+        hipStreamCreateWithFlags(&ctx->lanes[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->lanes[1], hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(BPX_E_HIP, "stream / counter allocation failed");
     }
@@ -848,7 +857,11 @@ int bpx_context_destroy(bpx_ctx* ctx) {
     if (!ctx) return BPX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->work_counter);
+    for (hipStream_t l : ctx->lanes)
+        if (l) {
+            (void)hipStreamSynchronize(l);
+            (void)hipStreamDestroy(l);
+        }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BPX_OK;
@@ -1135,7 +1148,7 @@ int bpx_result_wait(bpx_result* r) {
     bpx_ctx* ctx = r->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     for (;;) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(r->stream));
         const uint32_t ovf = (uint32_t)(r->h_meta[1] & 0xFFFFFFFFu);
         if (ovf & 2u) return fail(BPX_E_STATE, "internal: hit ring smaller than a lookup cell");
         if (!(ovf & 1u)) break;
@@ -1161,16 +1174,25 @@ int bpx_result_wait(bpx_result* r) {
         if (!r->h_records || !r->h_subjects || !r->h_queries)
             return fail(BPX_E_NOMEM, "pinned allocation for records failed");
         HIP_TRY(hipMemcpyAsync(r->h_records, r->d_records, r->total * sizeof(bpx_init_hsp), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        HIP_TRY(hipMemcpyAsync(r->h_subjects, r->d_subjects, r->total * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(r->h_queries, r->d_queries, r->total * 4, hipMemcpyDeviceToHost, ctx->stream));
+                               r->stream));
+        HIP_TRY(hipMemcpyAsync(r->h_subjects, r->d_subjects, r->total * 4, hipMemcpyDeviceToHost, r->stream));
+        HIP_TRY(hipMemcpyAsync(r->h_queries, r->d_queries, r->total * 4, hipMemcpyDeviceToHost, r->stream));
     }
-    HIP_TRY(hipEventRecord(r->ev[2], ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipEventRecord(r->ev[2], r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
     HIP_TRY(hipEventElapsedTime(&r->kernel_ms, r->ev[0], r->ev[1]));
     HIP_TRY(hipEventElapsedTime(&r->device_ms, r->ev[0], r->ev[2]));
     r->waited = true;
     return BPX_OK;
+}
+
+int bpx_result_ready(const bpx_result* r) {
+    if (!r) return fail(BPX_E_ARG, "result is NULL");
+    if (r->waited) return 1;
+    const hipError_t e = hipStreamQuery(r->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(BPX_E_HIP, "stream query: %s", hipGetErrorString(e));
 }
 
 int bpx_result_timing(const bpx_result* r, float* kernel_ms, float* device_ms) {
@@ -1256,16 +1278,16 @@ int bpx_result_replay(bpx_result* r) {
     r->waited = false;
     r->total = 0;
     if (r->graph_stale || !r->graph_exec) return graph_capture_and_launch(r);
-    HIP_TRY(hipEventRecord(r->ev[0], ctx->stream));
-    HIP_TRY(hipGraphLaunch(r->graph_exec, ctx->stream));
-    HIP_TRY(hipEventRecord(r->ev[1], ctx->stream));
+    HIP_TRY(hipEventRecord(r->ev[0], r->stream));
+    HIP_TRY(hipGraphLaunch(r->graph_exec, r->stream));
+    HIP_TRY(hipEventRecord(r->ev[1], r->stream));
     return BPX_OK;
 }
 
 int bpx_result_destroy(bpx_result* r) {
     if (!r) return BPX_OK;
     (void)hipSetDevice(r->ctx->device);
-    (void)hipStreamSynchronize(r->ctx->stream);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
     graph_release(r);
     result_release(r, false);
     for (auto& ev : r->ev)

@@ -186,6 +186,9 @@ int bpx_scan_batch(bpx_ctx *ctx, const bpx_batch *b, const bpx_db *db, uint32_t 
                    bpx_result **out);
 /* Blocks until the scan and its result copies are complete. */
 int bpx_result_wait(bpx_result *r);
+/* 1 if the scan's device work has finished (bpx_result_wait will not block
+ * on the device), 0 if not, < 0 on error.  Never blocks. */
+int bpx_result_ready(const bpx_result *r);
 /* Times of the last scan, measured with HIP events on the stream the
  * kernels run on: the word-finder kernels alone, and the whole device phase
  * (word finder + compaction + copies to the host). */
@@ -276,6 +279,47 @@ int bpx_costmodel_destroy(bpx_costmodel *cm);
  * longest-first order.  Scans with a window above 1,024 are refused while a
  * cost order is set.  Not while a scan of this database is in flight. */
 int bpx_db_set_cost(bpx_db *db, const float *cost);
+
+/* ---- query server -------------------------------------------------------
+ * No reference counterpart (blastp-cuda serves one search per process).
+ * Continuous batching over a resident database: submitted queries wait in a
+ * queue; a worker thread admits them into batched scans when a device slot
+ * is free (at most max_inflight batches, on two stream lanes) and the
+ * batch's result buffers fit hbm_budget -- at once when the device is idle,
+ * otherwise when max_batch queries wait or the oldest has waited
+ * max_wait_us.  Each request's records and stats are exactly those of
+ * scanning it alone. */
+typedef struct bpx_server bpx_server;
+typedef struct bpx_server_config {
+    int32_t max_batch;          /* queries per batch (0 = 256) */
+    int32_t max_inflight;       /* batches on the device at once, 1..2 (0 = 2) */
+    int32_t max_wait_us;        /* gathering time while a batch runs (0 = 10,000) */
+    int32_t reserved;
+    int64_t hbm_budget;         /* result-buffer bytes one batch may take (0 = 1/8 of HBM) */
+    bpx_word_params params;     /* word_length 0 = blastp defaults; diag_mask must be 0 */
+    bpx_batch_policy policy;
+    bpx_launch launch;
+} bpx_server_config;
+typedef struct bpx_request_info {
+    uint64_t num_records;
+    bpx_ungapped_stats stats;
+    double latency_ms;          /* submit -> result ready */
+    double queue_ms;            /* submit -> its batch launched */
+    uint32_t batch_size;        /* queries in its batch */
+    uint32_t reserved;
+} bpx_request_info;
+int bpx_server_create(bpx_ctx *ctx, const bpx_db *db, const bpx_server_config *cfg, bpx_server **out);
+int bpx_server_submit(bpx_server *s, const uint8_t *query, int32_t qlen, uint64_t *ticket);
+/* Waits up to timeout_ms (-1 = no limit; BPX_E_STATE on timeout); returns
+ * the request's own status. */
+int bpx_server_wait(bpx_server *s, uint64_t ticket, int32_t timeout_ms, bpx_request_info *info);
+/* Copies the request's records (ordered by subject, subject offset, query
+ * offset) and subject ids, then forgets the ticket. */
+int bpx_server_records(bpx_server *s, uint64_t ticket, bpx_init_hsp *records, uint32_t *subjects,
+                       uint64_t capacity);
+int bpx_server_counters(bpx_server *s, uint64_t *batches, uint64_t *completed);
+/* Finishes every queued request, then stops the worker. */
+int bpx_server_destroy(bpx_server *s);
 
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
 const int8_t *bpx_blosum62(void);
