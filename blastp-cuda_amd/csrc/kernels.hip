@@ -432,10 +432,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             if (lane == 0) k = atomicAdd(&ctl[1], 1u);
             k = uniform(__shfl(k, 0, 64));
             if (k >= w.count) break;
-            const uint32_t sid = p.order[w.first + k];
+            // descriptors are stored in processing order: consecutive
+            // subjects of an item read consecutive entries
+            const uint32_t opos = w.first + k;  // position in the processing order
+            const uint32_t sid = p.order[opos];
             const uint32_t slot = sid - p.first;
-            const uint8_t* __restrict__ S = p.residues + p.soff[sid];
-            const int32_t slen = (int32_t)p.slen[sid];
+            const uint8_t* __restrict__ S = p.residues + p.poff[opos];
+            const int32_t slen = (int32_t)p.plen[opos];
             const int32_t send = (slen + 15) & ~15;
 
             // ---- stage the subject (or its first window) into LDS ----
@@ -699,18 +702,25 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             }
 
             // ---- per-subject outputs ----
+            // Counts and run lengths were zero-filled before the launch, so
+            // only subjects with records write them; counters go to their
+            // processing-order position (sequential within an item) and are
+            // put in subject order after the scan.
             const uint32_t gslot = w.group * p.count + slot;
+            const bool has_run = seg_fill > rec_start;
             if (!MULTI) {
                 if (lane == 0) {
-                    const size_t i = (size_t)q0.qid * p.count + slot;
-                    p.cnt[i] = saved;
+                    if (saved) p.cnt[(size_t)q0.qid * p.count + slot] = saved;
                     if (p.stats) {
-                        p.stats[3 * i + 0] = (int32_t)hits;
-                        p.stats[3 * i + 1] = (int32_t)saved;
-                        p.stats[3 * i + 2] = (int32_t)extended;
+                        int32_t* st = p.stats + 3 * ((size_t)q0.qid * p.count + opos);
+                        st[0] = (int32_t)hits;
+                        st[1] = (int32_t)saved;
+                        st[2] = (int32_t)extended;
                     }
-                    p.run_loc[gslot] = seg_base + rec_start;
-                    p.run_len[gslot] = seg_fill - rec_start;
+                    if (has_run) {
+                        p.run_loc[gslot] = seg_base + rec_start;
+                        p.run_len[gslot] = seg_fill - rec_start;
+                    }
                 }
                 if (hits > 0) {  // Blast_UngappedStatsUpdate skips subjects without hits
                     a_hits += hits;
@@ -723,12 +733,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 wave_sync();
                 for (uint32_t jj = lane; jj < g.nq; jj += 64) {
                     QCount& cq = qc[jj];
-                    const size_t i = (size_t)qinfo[jj].qid * p.count + slot;
-                    p.cnt[i] = cq.saved;
+                    const size_t qrow = (size_t)qinfo[jj].qid * p.count;
+                    if (cq.saved) p.cnt[qrow + slot] = cq.saved;
                     if (p.stats) {
-                        p.stats[3 * i + 0] = (int32_t)cq.hits;
-                        p.stats[3 * i + 1] = (int32_t)cq.saved;
-                        p.stats[3 * i + 2] = (int32_t)cq.ext;
+                        int32_t* st = p.stats + 3 * (qrow + opos);
+                        st[0] = (int32_t)cq.hits;
+                        st[1] = (int32_t)cq.saved;
+                        st[2] = (int32_t)cq.ext;
                     }
                     if (cq.hits > 0) {
                         cq.agg_hits += cq.hits;
@@ -739,7 +750,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     }
                     cq.hits = cq.ext = cq.saved = 0;
                 }
-                if (lane == 0) {
+                if (lane == 0 && has_run) {
                     p.run_loc[gslot] = seg_base + rec_start;
                     p.run_len[gslot] = seg_fill - rec_start;
                 }
@@ -889,6 +900,18 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
     }
 }
 
+// per-(query, position) counters -> per-(query, subject) order
+__global__ void k_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
+                                uint32_t count, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t q = i / count, pos = i % count;
+        const uint64_t j = q * count + (order[pos] - first);
+        dst[3 * j + 0] = src[3 * i + 0];
+        dst[3 * j + 1] = src[3 * i + 1];
+        dst[3 * j + 2] = src[3 * i + 2];
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = value;
@@ -975,6 +998,15 @@ hipError_t launch_compact(const CompactParams& p, hipStream_t stream) {
         const uint64_t blocks = (runs + kCompactThreads - 1) / kCompactThreads;
         hipLaunchKernelGGL(k_gather_runs, dim3((unsigned)blocks), dim3(kCompactThreads), 0, stream, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
+                                uint32_t count, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_permute_stats, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, stream, src, dst,
+                       order, first, count, n);
     return hipGetLastError();
 }
 

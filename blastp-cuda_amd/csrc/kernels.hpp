@@ -60,6 +60,8 @@ struct ScanParams {
     const uint64_t* soff;
     const uint32_t* slen;
     const uint32_t* order;      // subject ids, processing order
+    const uint64_t* poff;       // arena offset of order[k]
+    const uint32_t* plen;       // length of order[k]
     uint32_t first;             // subject id of slot 0
     uint32_t count;             // subject slots
     // queries
@@ -75,10 +77,10 @@ struct ScanParams {
     uint32_t nitems;
     uint32_t* work_counter;
     // outputs
-    uint32_t* cnt;              // saved records per (qid, slot): qid * count + slot
-    int32_t* stats;             // optional: 3 per (qid, slot)
+    uint32_t* cnt;              // saved records per (qid, slot): qid * count + slot; zero-filled
+    int32_t* stats;             // optional: 3 per (qid, processing position): qid * count + k
     uint32_t* run_loc;          // per (group, slot): first segment record of the run
-    uint32_t* run_len;          // per (group, slot): records in the run
+    uint32_t* run_len;          // per (group, slot): records in the run; zero-filled
     unsigned long long* ustats; // 5 per qid (bpx_ungapped_stats order)
     SegRecord* seg;
     uint32_t seg_cap;           // records per wave segment
@@ -145,5 +147,8 @@ constexpr uint32_t kTile = 1024;
 hipError_t launch_compact(const CompactParams& p, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream);
+// counters per (query, processing position) -> per (query, subject slot)
+hipError_t launch_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
+                                uint32_t count, uint64_t n, hipStream_t stream);
 
 }  // namespace bpx

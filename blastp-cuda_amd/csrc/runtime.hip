@@ -46,7 +46,7 @@ int fail(int code, const char* fmt, ...) {
             return fail(BPX_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
     } while (0)
 
-constexpr uint32_t kArenaAlign = 16;   // subject starts in the arena
+constexpr uint32_t kArenaAlign = 128;  // subject starts in the arena: whole cache lines per subject
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
 constexpr int kMaxClasses = 64;        // launches per scan: (table placement, multi, waves) bins x slot width
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
@@ -132,6 +132,9 @@ struct bpx_db {
     uint64_t* d_off = nullptr;
     uint32_t* d_len = nullptr;
     uint32_t* d_order = nullptr;  // all subjects, longest first
+    uint64_t* d_poff = nullptr;   // arena offset of d_order[k]
+    uint32_t* d_plen = nullptr;   // length of d_order[k]
+    std::vector<uint64_t> h_off;
     std::vector<uint32_t> h_len;
     std::vector<uint32_t> h_order;
     bool cost_order = false;  // h_order set by bpx_db_set_cost
@@ -204,6 +207,9 @@ struct bpx_result {
     uint32_t* d_queries = nullptr;
     unsigned long long* d_meta = nullptr;  // [0] total, [1] overflow word, [8..] 5 stats per query
     uint32_t* d_order = nullptr;           // owned only for sub-range scans
+    uint64_t* d_poff = nullptr;            // (sub-range) descriptors in processing order
+    uint32_t* d_plen = nullptr;
+    int32_t* d_stats_k = nullptr;          // counters per (query, processing position)
     bpx::WorkItem* d_items = nullptr;
     // host (pinned)
     unsigned long long* h_meta = nullptr;
@@ -501,6 +507,8 @@ void result_release(bpx_result* r, bool keep_order) {
     auto& pinned = r->ctx->pinned;
     pool.put(r->d_cnt);
     pool.put(r->d_stats);
+    pool.put(r->d_stats_k);
+    r->d_stats_k = nullptr;
     pool.put(r->d_run_loc);
     pool.put(r->d_run_len);
     pool.put(r->d_seg);
@@ -514,7 +522,11 @@ void result_release(bpx_result* r, bool keep_order) {
     r->d_work = nullptr;
     if (!keep_order) {
         pool.put(r->d_order);
+        pool.put(r->d_poff);
+        pool.put(r->d_plen);
         r->d_order = nullptr;
+        r->d_poff = nullptr;
+        r->d_plen = nullptr;
     }
     pinned.put(r->h_records);
     pinned.put(r->h_subjects);
@@ -555,7 +567,10 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     if (r->seg_total > 0xFFFFFFFFull) return fail(BPX_E_BUDGET, "record segments exceed 2^32 records");
     const uint64_t ntiles = (pairs + bpx::kTile - 1) / bpx::kTile;
     r->d_cnt = (uint32_t*)pool.get(pairs * 4);
-    if (r->flags & BPX_OUT_SUBJECT_STATS) r->d_stats = (int32_t*)pool.get(pairs * 12);
+    if (r->flags & BPX_OUT_SUBJECT_STATS) {
+        r->d_stats = (int32_t*)pool.get(pairs * 12);
+        r->d_stats_k = (int32_t*)pool.get(pairs * 12);
+    }
     r->d_run_loc = (uint32_t*)pool.get(runs * 4);
     r->d_run_len = (uint32_t*)pool.get(runs * 4);
     r->d_seg = (bpx::SegRecord*)pool.get(r->seg_total * sizeof(bpx::SegRecord));
@@ -567,7 +582,8 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
     r->d_work = (uint32_t*)pool.get(kMaxClasses * 4);
     r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
-    if (!r->d_cnt || ((r->flags & BPX_OUT_SUBJECT_STATS) && !r->d_stats) || !r->d_run_loc || !r->d_run_len ||
+    if (!r->d_cnt || ((r->flags & BPX_OUT_SUBJECT_STATS) && (!r->d_stats || !r->d_stats_k)) || !r->d_run_loc ||
+        !r->d_run_len ||
         !r->d_seg || !r->d_tile || !r->d_records || !r->d_subjects || !r->d_queries || !r->d_meta || !r->d_items ||
         !r->d_work ||
         !r->h_meta)
@@ -589,6 +605,9 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
     hipStream_t st = r->stream;
     HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_meta), 0u, 2 * (kMetaWords + 5ull * set.nq), st));
     HIP_TRY(bpx::launch_fill_u32(r->d_work, 0u, kMaxClasses, st));
+    // the kernel writes counts and run lengths only for subjects with records
+    HIP_TRY(bpx::launch_fill_u32(r->d_cnt, 0u, (uint64_t)set.nq * r->count, st));
+    HIP_TRY(bpx::launch_fill_u32(r->d_run_len, 0u, (uint64_t)set.groups.size() * r->count, st));
     unsigned long long* ustats = r->d_meta + kMetaWords;
     uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
     if (events) HIP_TRY(hipEventRecord(r->ev[0], st));
@@ -601,6 +620,8 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
         sp.soff = db->d_off;
         sp.slen = db->d_len;
         sp.order = order;
+        sp.poff = r->d_poff ? r->d_poff : db->d_poff;
+        sp.plen = r->d_plen ? r->d_plen : db->d_plen;
         sp.first = r->first;
         sp.count = r->count;
         sp.groups = set.d_groups;
@@ -615,7 +636,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
         sp.nitems = c.nitems;
         sp.work_counter = r->d_work + ci;
         sp.cnt = r->d_cnt;
-        sp.stats = r->d_stats;
+        sp.stats = r->d_stats_k;
         sp.run_loc = r->d_run_loc;
         sp.run_len = r->d_run_len;
         sp.ustats = ustats;
@@ -669,6 +690,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
         const uint64_t pairs = (uint64_t)set.nq * r->count;
         if (!r->h_sstats) r->h_sstats = (bpx_subject_stats*)ctx->pinned.get(pairs * sizeof(bpx_subject_stats));
         if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
+        HIP_TRY(bpx::launch_permute_stats(r->d_stats_k, r->d_stats, order, r->first, r->count, pairs, st));
         HIP_TRY(hipMemcpyAsync(r->h_sstats, r->d_stats, pairs * sizeof(bpx_subject_stats), hipMemcpyDeviceToHost, st));
     }
     return BPX_OK;
@@ -788,13 +810,25 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
         sub.reserve(count);
         for (uint32_t id : db->h_order)
             if (id >= first && id - first < count) sub.push_back(id);
+        std::vector<uint64_t> soff(sub.size());
+        std::vector<uint32_t> slen(sub.size());
+        for (size_t k = 0; k < sub.size(); ++k) {
+            soff[k] = db->h_off[sub[k]];
+            slen[k] = db->h_len[sub[k]];
+        }
         r->d_order = (uint32_t*)ctx->pool.get(std::max<size_t>(count, 1) * 4);
-        if (!r->d_order) {
+        r->d_poff = (uint64_t*)ctx->pool.get(std::max<size_t>(count, 1) * 8);
+        r->d_plen = (uint32_t*)ctx->pool.get(std::max<size_t>(count, 1) * 4);
+        if (!r->d_order || !r->d_poff || !r->d_plen) {
             bpx_result_destroy(r);
             return fail(BPX_E_NOMEM, "work list allocation failed");
         }
         if (count) {
             hipError_t e = hipMemcpyAsync(r->d_order, sub.data(), (size_t)count * 4, hipMemcpyHostToDevice, r->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(r->d_poff, soff.data(), (size_t)count * 8, hipMemcpyHostToDevice, r->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(r->d_plen, slen.data(), (size_t)count * 4, hipMemcpyHostToDevice, r->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
             if (e != hipSuccess) {
                 bpx_result_destroy(r);
@@ -812,6 +846,24 @@ int start_scan(bpx_ctx* ctx, const QuerySet& set, const bpx_db* db, uint32_t fir
     return BPX_OK;
 }
 
+}  // namespace
+
+namespace {
+// the processing order and the subject descriptors in that order
+hipError_t upload_order(bpx_db* db) {
+    const size_t n = db->n;
+    if (n == 0) return hipSuccess;
+    std::vector<uint64_t> poff(n);
+    std::vector<uint32_t> plen(n);
+    for (size_t k = 0; k < n; ++k) {
+        poff[k] = db->h_off[db->h_order[k]];
+        plen[k] = db->h_len[db->h_order[k]];
+    }
+    hipError_t e = hipMemcpy(db->d_order, db->h_order.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db->d_poff, poff.data(), n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db->d_plen, plen.data(), n * 4, hipMemcpyHostToDevice);
+    return e;
+}
 }  // namespace
 
 extern "C" {
@@ -915,7 +967,8 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
                      [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
     const size_t n1 = std::max<uint32_t>(num_subjects, 1);
     if (hipMalloc(&db->d_arena, arena) != hipSuccess || hipMalloc(&db->d_off, n1 * 8) != hipSuccess ||
-        hipMalloc(&db->d_len, n1 * 4) != hipSuccess || hipMalloc(&db->d_order, n1 * 4) != hipSuccess) {
+        hipMalloc(&db->d_len, n1 * 4) != hipSuccess || hipMalloc(&db->d_order, n1 * 4) != hipSuccess ||
+        hipMalloc(&db->d_poff, n1 * 8) != hipSuccess || hipMalloc(&db->d_plen, n1 * 4) != hipSuccess) {
         bpx_db_destroy(db);
         return fail(BPX_E_NOMEM, "device allocation of %llu-byte arena failed", (unsigned long long)arena);
     }
@@ -923,9 +976,9 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
     if (e == hipSuccess && num_subjects) {
         e = hipMemcpy(db->d_off, off.data(), (size_t)num_subjects * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(db->d_len, lengths, (size_t)num_subjects * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(db->d_order, db->h_order.data(), (size_t)num_subjects * 4, hipMemcpyHostToDevice);
     }
+    db->h_off = std::move(off);
+    if (e == hipSuccess && num_subjects) e = upload_order(db);
     if (e != hipSuccess) {
         bpx_db_destroy(db);
         return fail(BPX_E_HIP, "arena upload failed: %s", hipGetErrorString(e));
@@ -953,8 +1006,8 @@ int bpx_db_set_cost(bpx_db* db, const float* cost) {
     } else {
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
     }
-    if (db->n) HIP_TRY(hipMemcpy(db->d_order, order.data(), (size_t)db->n * 4, hipMemcpyHostToDevice));
     db->h_order.swap(order);
+    HIP_TRY(upload_order(db));
     db->cost_order = cost != nullptr;
     return BPX_OK;
 }
@@ -1002,6 +1055,8 @@ int bpx_db_destroy(bpx_db* db) {
     (void)hipFree(db->d_off);
     (void)hipFree(db->d_len);
     (void)hipFree(db->d_order);
+    (void)hipFree(db->d_poff);
+    (void)hipFree(db->d_plen);
     delete db;
     return BPX_OK;
 }
