@@ -60,6 +60,8 @@ struct Slot<true> {
     static constexpr uint32_t kVal = 0x3FFFu;
 };
 constexpr int32_t kOffsetLimit = 1 << 28;
+constexpr uint32_t kSeg = 8;       // lanes per cooperative extension task
+constexpr uint32_t kCoopMax = 16;  // flushes of at most this many tasks (of a resident subject) run cooperatively
 constexpr int kMatrixStride = 32;        // LDS score rows padded 28 -> 32
 constexpr uint32_t kCtlBytes = 16;       // workgroup control words at LDS offset 0
 
@@ -220,42 +222,60 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     return r;
 }
 
-// Inclusive prefix sum / maximum inside each 16-lane row (DPP row_shr
-// 1, 2, 4, 8; lanes shifted in from outside the row contribute the identity).
-__device__ __forceinline__ int32_t row_incl_sum(int32_t v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+// Inclusive prefix sum / maximum inside each W-lane segment (W = 8 or 16) of
+// the wave: DPP row_shr 1, 2, 4 (and 8 for W = 16); a lane whose source lies
+// in the previous segment (or outside the 16-lane row) takes the identity.
+template <int W>
+__device__ __forceinline__ int32_t seg_incl_sum(int32_t v, uint32_t rl) {
+    int32_t t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += rl >= 1 ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += rl >= 2 ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += rl >= 4 ? t : 0;
+    if (W == 16) {
+        t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+        v += rl >= 8 ? t : 0;
+    }
     return v;
 }
-__device__ __forceinline__ int32_t row_incl_max(int32_t v) {
+template <int W>
+__device__ __forceinline__ int32_t seg_incl_max(int32_t v, uint32_t rl) {
     constexpr int32_t lo = (int32_t)0x80000000;
-    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xf, false));
+    int32_t t;
+    t = __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false);
+    v = rl >= 1 ? max(v, t) : v;
+    t = __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false);
+    v = rl >= 2 ? max(v, t) : v;
+    t = __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xf, false);
+    v = rl >= 4 ? max(v, t) : v;
+    if (W == 16) {
+        t = __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xf, false);
+        v = rl >= 8 ? max(v, t) : v;
+    }
     return v;
 }
 
-// One X-drop pass of a row: 16 consecutive steps per iteration.  Step k of
-// the row has score m_k; the running score is carry + prefix sum, the
-// running maximum the prefix maximum, and the first lane whose stop test
-// holds (or the first step past the end) ends the pass -- the sequential
-// loop of gpu_blastp_kernel.cu:451-474 (left) / :523-541 (right), evaluated
-// 16 steps at a time.  RIGHT adds the stop on a non-positive running score.
-// Returns (via refs) the maximum, the step of its first occurrence (-1 if it
-// never rose above the start value) and the stop step.
-template <bool RIGHT>
-__device__ __forceinline__ void row_xdrop(const int8_t* M, const uint8_t* Q, uint32_t qcap, const uint8_t* sbuf,
+// One X-drop pass of a W-lane segment: W consecutive steps per iteration.
+// Step k of the segment has score m_k; the running score is carry + prefix
+// sum, the running maximum the prefix maximum, and the first lane whose stop
+// test holds (or the first step past the end) ends the pass -- the
+// sequential loop of gpu_blastp_kernel.cu:451-474 (left) / :523-541 (right),
+// evaluated W steps at a time.  RIGHT adds the stop on a non-positive running
+// score.  Returns (via refs) the maximum, the step of its first occurrence
+// (-1 if it never rose above the start value) and the stop step.
+template <bool RIGHT, int W>
+__device__ __forceinline__ void seg_xdrop(const int8_t* M, const uint8_t* Q, uint32_t qcap, const uint8_t* sbuf,
                                           uint32_t smask, int32_t cq, int32_t cs, int32_t nsteps, int32_t start,
                                           int32_t xdrop, uint32_t lane, int32_t& out_max, int32_t& out_best,
                                           int32_t& out_stop) {
     constexpr int32_t kNeg = -(1 << 22);
-    const uint32_t rl = lane & 15u, rbase = lane & ~15u;
+    constexpr uint32_t kSegMask = (1u << W) - 1u;
+    const uint32_t rl = lane & (W - 1u), rbase = lane & ~(W - 1u);
     int32_t carry = start, mx = start, best = -1, stop_at = nsteps;
     bool done = nsteps <= 0;
-    for (int32_t k0 = 0; __ballot(!done) != 0; k0 += 16) {
+    for (int32_t k0 = 0; __ballot(!done) != 0; k0 += W) {
         const int32_t k = k0 + (int32_t)rl;
         const bool ok = !done && k < nsteps;
         // left steps walk down from (cq, cs), right steps walk up from it
@@ -264,14 +284,14 @@ __device__ __forceinline__ void row_xdrop(const int8_t* M, const uint8_t* Q, uin
         const uint32_t qb = Q[min((uint32_t)max(qi, 0), qcap - 1)];
         const uint32_t sb = sbuf[(uint32_t)max(si, 0) & smask];
         const int32_t m = ok ? (int32_t)M[qb * kMatrixStride + sb] : kNeg;
-        const int32_t run = carry + row_incl_sum(m);
-        const int32_t pm = max(mx, row_incl_max(run));
+        const int32_t run = carry + seg_incl_sum<W>(m, rl);
+        const int32_t pm = max(mx, seg_incl_max<W>(run, rl));
         const bool stop = ok ? ((RIGHT && run <= 0) || pm - run >= xdrop) : true;
-        const uint32_t smaskr = (uint32_t)(__ballot(stop) >> rbase) & 0xFFFFu;
-        const uint32_t kb = smaskr ? (uint32_t)__ffs(smaskr) - 1u : 15u;
+        const uint32_t smaskr = (uint32_t)(__ballot(stop) >> rbase) & kSegMask;
+        const uint32_t kb = smaskr ? (uint32_t)__ffs(smaskr) - 1u : W - 1u;
         const int32_t pm_kb = __shfl(pm, (int)(rbase + kb), 64);
-        const int32_t run_last = __shfl(run, (int)(rbase + 15u), 64);
-        const uint32_t eq = (uint32_t)(__ballot(run == pm_kb && rl <= kb) >> rbase) & 0xFFFFu;
+        const int32_t run_last = __shfl(run, (int)(rbase + W - 1u), 64);
+        const uint32_t eq = (uint32_t)(__ballot(run == pm_kb && rl <= kb) >> rbase) & kSegMask;
         if (!done) {
             if (pm_kb > mx) {
                 mx = pm_kb;
@@ -289,10 +309,10 @@ __device__ __forceinline__ void row_xdrop(const int8_t* M, const uint8_t* Q, uin
     out_stop = stop_at;
 }
 
-// s_BlastAaExtendTwoHit for one task per 16-lane row (all 64 lanes call
-// it; a row with valid == false does no work).  Same result as
+// s_BlastAaExtendTwoHit for one task per W-lane segment (all 64 lanes call
+// it; a segment with valid == false does no work).  Same result as
 // extend_two_hit for a subject resident in sbuf.
-template <int WORD>
+template <int WORD, int W>
 __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uint8_t* Q, uint32_t qcap,
                                                       const uint8_t* sbuf, uint32_t smask, int32_t slen, bool valid,
                                                       int32_t qstart, int32_t qlen, int32_t s_left, int32_t s,
@@ -310,13 +330,13 @@ __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uin
     // left: steps k = 0..n at (cr-1-k, sr-1-k)
     const int32_t nl = valid ? min(sr - 1, qr - 1) + 1 : 0;
     int32_t lmax, lbest, lstop;
-    row_xdrop<false>(M, Q, qcap, sbuf, smask, cr - 1, sr - 1, nl, 0, xdrop, lane, lmax, lbest, lstop);
+    seg_xdrop<false, W>(M, Q, qcap, sbuf, smask, cr - 1, sr - 1, nl, 0, xdrop, lane, lmax, lbest, lstop);
     const int32_t left_len = lbest + 1;
     TwoHit r;
     r.right_extend = valid && left_len >= sr - s_left;
     const int32_t nr = r.right_extend ? min(slen - sr, qlen - qr) : 0;
     int32_t rmax, rbest, rstop;
-    row_xdrop<true>(M, Q, qcap, sbuf, smask, cr, sr, nr, lmax, xdrop, lane, rmax, rbest, rstop);
+    seg_xdrop<true, W>(M, Q, qcap, sbuf, smask, cr, sr, nr, lmax, xdrop, lane, rmax, rbest, rstop);
     r.s_last = r.right_extend ? sr + rstop : sr;
     r.hsp_q = qr - left_len;
     r.hsp_s = sr - left_len;
@@ -529,17 +549,18 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             };
 
             // Runs the queued extensions and writes their outcome to the
-            // diagonal table.  A few tasks run on 16-lane rows (4 per pass);
-            // more run one per lane.
+            // diagonal table.  A few tasks run on kSeg-lane segments
+            // (64 / kSeg per pass); more run one per lane.
             auto flush = [&]() {
                 const uint32_t nt = ntasks;
                 wave_sync();
-                const bool coop = resident && nt <= 16;
-                const uint32_t npass = coop ? (nt + 3) / 4 : (nt + 63) / 64;
+                constexpr uint32_t kPer = 64u / kSeg;
+                const bool coop = resident && nt <= kCoopMax;
+                const uint32_t npass = coop ? (nt + kPer - 1) / kPer : (nt + 63) / 64;
                 for (uint32_t pass = 0; pass < npass; ++pass) {
-                    const uint32_t t = coop ? pass * 4 + (lane >> 4) : pass * 64 + lane;
+                    const uint32_t t = coop ? pass * kPer + lane / kSeg : pass * 64 + lane;
                     const bool tvalid = t < nt;
-                    const bool mine = coop ? (tvalid && (lane & 15u) == 0) : tvalid;
+                    const bool mine = coop ? (tvalid && (lane & (kSeg - 1u)) == 0) : tvalid;
                     const uint32_t ti = min(t, kTaskCap - 1);
                     const uint32_t sc = tvalid ? t_sc[ti] : 0u;  // slot 0 / position 0 for idle lanes
                     const uint32_t tslot = sc & 0xFFFFu;
@@ -551,7 +572,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     const QueryInfo qi = MULTI ? qinfo[tj] : q0;
                     TwoHit x{};
                     if (coop) {
-                        x = coop_extend_two_hit<WORD>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid, (int32_t)qi.qstart,
+                        x = coop_extend_two_hit<WORD, kSeg>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid, (int32_t)qi.qstart,
                                                       (int32_t)qi.qlen, tsl, ts, tc, qi.xdrop, lane);
                     } else if (mine) {
                         if (resident)
