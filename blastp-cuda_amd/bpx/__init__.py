@@ -146,6 +146,15 @@ _sig("bpx_server_wait", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(RequestIn
 _sig("bpx_server_records", C.c_int, _vp, C.c_uint64, _vp, _vp, C.c_uint64)
 _sig("bpx_server_counters", C.c_int, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
 _sig("bpx_server_destroy", C.c_int, _vp)
+_sig("bpx_volume_read", C.c_int, C.c_char_p, C.POINTER(_vp))
+_sig("bpx_volume_info", C.c_int, _vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+     C.POINTER(C.c_char_p))
+_sig("bpx_volume_arrays", C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp))
+_sig("bpx_volume_destroy", C.c_int, _vp)
+_sig("bpx_volume_write", C.c_int, C.c_char_p, C.c_char_p, _vp, _vp, _vp, C.c_uint32)
+_sig("bpx_db_volumes", C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32))
+_sig("bpx_db_load_volumes", C.c_int, _vp, C.c_char_p, C.POINTER(_vp))
+_sig("bpx_volume_last_error", C.c_char_p)
 _sig("bpx_db_features", C.c_int, _vp, _vp, _vp, C.c_int32, _vp)
 _sig("bpx_db_set_cost", C.c_int, _vp, _vp)
 _sig("bpx_costmodel_create", C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp))
@@ -693,3 +702,61 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _vchk(rc):
+    if rc:
+        raise BpxError(rc, _lib.bpx_volume_last_error().decode())
+
+
+def volume_write(base: str, residues, offsets, lengths, title: str = ""):
+    """Writes an NCBI version-4 protein volume base.pin / .psq / .phr."""
+    r = np.ascontiguousarray(residues, dtype=np.uint8)
+    o = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+    _vchk(_lib.bpx_volume_write(base.encode(), title.encode(), _ptr(r), _ptr(o), _ptr(ln), len(ln)))
+
+
+def volume_read(base: str):
+    """Reads base.pin / base.psq: (residues, offsets[n+1], lengths[n], title)."""
+    h = _vp()
+    _vchk(_lib.bpx_volume_read(base.encode(), C.byref(h)))
+    try:
+        n, tot, mx, title = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_char_p()
+        _vchk(_lib.bpx_volume_info(h, C.byref(n), C.byref(tot), C.byref(mx), C.byref(title)))
+        rp, op, lp = _vp(), _vp(), _vp()
+        _vchk(_lib.bpx_volume_arrays(h, C.byref(rp), C.byref(op), C.byref(lp)))
+        n, tot = int(n.value), int(tot.value)
+
+        def arr(p, count, dt):
+            if count == 0 or not p.value:
+                return np.zeros(count, dtype=dt)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(count,)).copy()
+
+        return arr(rp, tot, np.uint8), arr(op, n + 1, np.uint64), arr(lp, n, np.uint32), title.value.decode()
+    finally:
+        _lib.bpx_volume_destroy(h)
+
+
+def db_volumes(base: str):
+    need, cnt = C.c_size_t(), C.c_uint32()
+    _vchk(_lib.bpx_db_volumes(base.encode(), None, 0, C.byref(need), C.byref(cnt)))
+    buf = C.create_string_buffer(need.value)
+    _vchk(_lib.bpx_db_volumes(base.encode(), buf, need.value, None, None))
+    return buf.value.decode().split("\n") if cnt.value else []
+
+
+def load_volumes(ctx: "Context", base: str) -> "Database":
+    """Every volume of an NCBI protein database (alias or single volume) as one
+    resident arena; subject i is OID i."""
+    db = Database.__new__(Database)
+    db.ctx = ctx
+    db._h = _vp()
+    rc = _lib.bpx_db_load_volumes(ctx._h, base.encode(), C.byref(db._h))
+    if rc:
+        msg = _lib.bpx_volume_last_error().decode() or _lib.bpx_last_error().decode()
+        raise BpxError(rc, msg)
+    n, nres, ab = C.c_uint32(), C.c_uint64(), C.c_uint64()
+    _check(_lib.bpx_db_info(db._h, C.byref(n), C.byref(nres), C.byref(ab)))
+    db.num_subjects, db.num_residues, db.arena_bytes = int(n.value), int(nres.value), int(ab.value)
+    return db

@@ -321,6 +321,31 @@ int bpx_server_counters(bpx_server *s, uint64_t *batches, uint64_t *completed);
 /* Finishes every queued request, then stops the worker. */
 int bpx_server_destroy(bpx_server *s);
 
+/* ---- NCBI protein database volumes (format version 4) --------------------
+ * base is a path without extension.  bpx_volume_read reads base.pin and
+ * base.psq: the index header walk follows the reference's reader
+ * (functions.cpp:181-215: version, type == 1 (protein), title, date, number
+ * of sequences), then the residue count, longest length and the sequence
+ * offsets; sequence i is [off[i], off[i+1] - 1) of .psq.  bpx_db_volumes
+ * lists a database's volumes -- the DBLIST line of base.pal
+ * (functions.cpp:217-353) or base itself.  bpx_db_load_volumes loads them
+ * all, in order, as one resident arena (subject i = OID i).
+ * bpx_volume_write writes the same layout (empty .phr headers).  Host-only
+ * except bpx_db_load_volumes. */
+typedef struct bpx_volume bpx_volume;
+int bpx_volume_read(const char *base, bpx_volume **out);
+int bpx_volume_info(const bpx_volume *v, uint32_t *num_sequences, uint64_t *num_residues,
+                    uint32_t *max_length, const char **title);
+int bpx_volume_arrays(const bpx_volume *v, const uint8_t **residues, const uint64_t **offsets,
+                      const uint32_t **lengths);
+int bpx_volume_destroy(bpx_volume *v);
+int bpx_volume_write(const char *base, const char *title, const uint8_t *residues,
+                     const uint64_t *offsets, const uint32_t *lengths, uint32_t n);
+int bpx_db_volumes(const char *base, char *names, size_t capacity, size_t *needed,
+                   uint32_t *count);
+int bpx_db_load_volumes(bpx_ctx *ctx, const char *base, bpx_db **out);
+const char *bpx_volume_last_error(void);
+
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
 const int8_t *bpx_blosum62(void);
 

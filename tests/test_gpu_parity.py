@@ -377,3 +377,32 @@ def test_graph_replay_equals_eager(ctx):
     g.close()
     b.close()
     ddb.close()
+
+
+def test_volumes_load_as_one_arena(ctx, tmp_path):
+    """An alias of three NCBI volumes loads as one arena (subject i = OID i)
+    and scans exactly like the same subjects given as arrays."""
+    q = bpx.synth_query(60, 0, 420)
+    spec = dict(seed=61, num_subjects=3000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
+                homolog_fraction=0.03, identity_min=0.4, identity_max=0.7, indel_min=0.0, indel_max=0.02)
+    db = bpx.synth_db(spec, [q], nthreads=THREADS)
+    cuts = [0, 1000, 1001, 3000]
+    names = []
+    for i in range(3):
+        a, b = cuts[i], cuts[i + 1]
+        o = db.offsets[a:b + 1] - db.offsets[a]
+        names.append(f"syn.{i:02d}")
+        bpx.volume_write(str(tmp_path / names[-1]), db.residues[db.offsets[a]:db.offsets[b]], o[:-1],
+                         db.lengths[a:b])
+    (tmp_path / "syn.pal").write_text("TITLE synthetic\nDBLIST " + " ".join(names) + "\n")
+    vdb = bpx.load_volumes(ctx, str(tmp_path / "syn"))
+    assert vdb.num_subjects == db.num_subjects and vdb.num_residues == db.num_residues
+    dq = ctx.prepare_query(q)
+    r = bpx.scan(ctx, dq, vdb).wait()
+    recs, subs = r.records()
+    st = r.subject_stats()
+    _, ustats = r.summary()
+    r.close()
+    assert_parity((st, recs, subs, ustats, None), oracle_run(q, db.residues, db.offsets, db.lengths))
+    dq.close()
+    vdb.close()
