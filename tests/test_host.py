@@ -120,3 +120,15 @@ def test_library_exports_every_declared_symbol():
     assert len(names) > 20
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_package_fails_loudly_without_the_library(tmp_path):
+    """No CPU fallback: the binding refuses to import when libbpx.so is absent."""
+    import shutil
+    import subprocess
+    import sys as _sys
+    pkg = tmp_path / "bpx"
+    pkg.mkdir()
+    shutil.copy(os.path.join(os.path.dirname(bpx.__file__), "__init__.py"), pkg / "__init__.py")
+    r = subprocess.run([_sys.executable, "-c", "import bpx"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode != 0 and "ImportError" in r.stderr and "not built" in r.stderr
