@@ -288,7 +288,8 @@ int bpx_db_set_cost(bpx_db *db, const float *cost);
  * batch's result buffers fit hbm_budget -- at once when the device is idle,
  * otherwise when max_batch queries wait or the oldest has waited
  * max_wait_us.  Each request's records and stats are exactly those of
- * scanning it alone. */
+ * scanning it alone.  While the server runs, its worker thread owns the
+ * context: make no other calls on that context from other threads. */
 typedef struct bpx_server bpx_server;
 typedef struct bpx_server_config {
     int32_t max_batch;          /* queries per batch (0 = 256) */
