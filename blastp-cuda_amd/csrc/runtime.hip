@@ -50,7 +50,7 @@ constexpr uint32_t kArenaAlign = 128;  // subject starts in the arena: whole cac
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
 constexpr int kMaxClasses = 64;        // launches per scan: (table placement, multi, waves) bins x slot width
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
-constexpr uint32_t kSegFloor = 256;    // smallest per-wave record segment of a first scan
+constexpr uint32_t kSegFloor = 1024;   // smallest per-wave record segment (64 B per record slot)
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
 // do not call hipMalloc/hipFree.
@@ -598,7 +598,7 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
 // the word-finder launches with the timing events (not when captured into a
 // graph: then the caller brackets the graph launch).  The work items must be
 // on the device already (upload_items).
-int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
+int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start_event = true) {
     bpx_ctx* ctx = r->ctx;
     const QuerySet& set = *r->set;
     const bpx_db* db = r->db;
@@ -610,7 +610,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events) {
     HIP_TRY(bpx::launch_fill_u32(r->d_run_len, 0u, (uint64_t)set.groups.size() * r->count, st));
     unsigned long long* ustats = r->d_meta + kMetaWords;
     uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
-    if (events) HIP_TRY(hipEventRecord(r->ev[0], st));
+    if (events && start_event) HIP_TRY(hipEventRecord(r->ev[0], st));
     uint64_t seg_at = 0;
     for (size_t ci = 0; ci < r->classes.size(); ++ci) {
         const auto& c = r->classes[ci];
@@ -703,9 +703,11 @@ int upload_items(bpx_result* r) {
     return BPX_OK;
 }
 
-int enqueue(bpx_result* r, const uint32_t* order) {
+// start_event = false keeps the scan's first start event: a rescan's time
+// then includes the launch whose segments overflowed
+int enqueue(bpx_result* r, const uint32_t* order, bool start_event = true) {
     int rc = upload_items(r);
-    return rc ? rc : enqueue_device(r, order, true);
+    return rc ? rc : enqueue_device(r, order, true, start_event);
 }
 
 void graph_release(bpx_result* r) {
@@ -1216,7 +1218,7 @@ int bpx_result_wait(bpx_result* r) {
         result_release(r, true);
         int rc = alloc_buffers(r, cap);
         if (rc) return rc;
-        rc = enqueue(r, order);
+        rc = enqueue(r, order, false);
         if (rc) return rc;
         r->graph_stale = r->graph != nullptr;
         ++r->reruns;
