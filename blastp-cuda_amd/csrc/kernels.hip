@@ -503,10 +503,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     if (active && fresh) atomicAdd(&qc[j].hits, 1u);
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
-                const uint32_t h = dslot & (p.claim_slots - 1);
+                // The first round hashes the slot by its low bits; later rounds
+                // by a multiplicative hash, so lanes that lost only to a hash
+                // collision rarely collide again.  Lanes on one slot share
+                // every round's hash, so the lowest still goes first.
+                const uint32_t cmask = p.claim_slots - 1;
+                uint32_t h = dslot & cmask;
+                const uint32_t h2 = ((dslot * 0x9E3779B1u) >> 16) & cmask;
                 bool deferred = false;
                 const int32_t sd = s + doff;
-                while (__ballot(active) != 0) {
+                for (bool first = true; __ballot(active) != 0; first = false) {
+                    if (!first) h = h2;
                     if (active) atomicMin(&claim[h], lane);
                     wave_sync();
                     // slot state and claim are read by every lane, together;
