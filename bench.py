@@ -25,7 +25,7 @@ compared with the static longest-first order.
 checks every result against the plain single-query scan.
 
 Inputs are resident in HBM before the timed region (database arena and query
-tables uploaded once).  Prints ONE JSON line (the driver's contract); extra
+tables uploaded once).  Prints ONE JSON line; extra
 detail goes to stderr.
 """
 import argparse
@@ -47,7 +47,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def config2(threads):
+def plain_workload(threads):
     import bpx
     q = bpx.synth_query(0, 0, 500)
     spec = dict(seed=2, num_subjects=2_000_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
@@ -55,7 +55,7 @@ def config2(threads):
     return q, bpx.synth_db(spec, [q], nthreads=threads)
 
 
-def config3(threads):
+def batched_workload(threads):
     import bpx
     lens = bpx.synth_query_lengths(3, 1024, "loguniform", 50, 2000)
     qs = [bpx.synth_query(3, i, int(n)) for i, n in enumerate(lens)]
@@ -134,9 +134,9 @@ def main():
         return serving(args)
 
     import bpx
-    log("generating config-2 workload ...")
+    log("generating the 500-aa x 2,000,000-subject workload ...")
     t0 = time.perf_counter()
-    q, db = config2(args.threads)
+    q, db = plain_workload(args.threads)
     log(f"  {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256} "
         f"({time.perf_counter() - t0:.1f} s)")
     ctx = bpx.Context(0)
@@ -176,7 +176,7 @@ def main():
                 "kernel_ms_p50": round(float(np.percentile(kernel_ms, 50)), 4),
                 "kernel_ms_p99": round(float(np.percentile(kernel_ms, 99)), 4),
                 "bytes_alg": int(bytes_alg)}
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_scan_config2.json")
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_k_scan_plain_500aa_2M_subjects.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("bytes_per_launch")
@@ -190,7 +190,7 @@ def main():
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
            "data": "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)",
-           "config": {"workload": "plain: 1 query x 500 aa vs 2,000,000-subject DB (config 2)",
+           "config": {"workload": "plain: 1 query x 500 aa vs 2,000,000-subject DB",
                       "queries": 1, "query_len": 500, "subjects": db.num_subjects,
                       "residues": db.num_residues, "db_sha256": db.sha256,
                       "records": int(nrec), "lookup_hits": ust["lookup_hits"],
@@ -225,7 +225,7 @@ def batched(args):
     import bpx
     log("generating the 1,024-query batched workload ...")
     t0 = time.perf_counter()
-    qs, db = config3(args.threads)
+    qs, db = batched_workload(args.threads)
     qres = int(sum(len(q) for q in qs))
     log(f"  {len(qs)} queries ({qres} residues), {db.num_subjects} subjects, {db.num_residues} residues, "
         f"sha256 {db.sha256} ({time.perf_counter() - t0:.1f} s)")
@@ -295,7 +295,7 @@ def batched(args):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
            "data": "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)",
-           "config": {"workload": "batched: 1,024 queries (log-uniform 50-2,000 aa) vs 500,000-subject DB (config 3)",
+           "config": {"workload": "batched: 1,024 queries (log-uniform 50-2,000 aa) vs 500,000-subject DB",
                       "queries": len(qs), "query_residues": qres, "subjects": db.num_subjects,
                       "residues": db.num_residues, "db_sha256": db.sha256, "records": int(nrec),
                       "groups": passes, "group_residues": gr, "group_queries": gq, "launch": launch,
@@ -366,8 +366,7 @@ def skewed(args):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_model, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
            "data": "synthetic (seeded generator: skewed lengths, 20 % biased composition with tandem repeats)",
-           "config": {"workload": "batched + learned-cost binning: 1,024 queries vs skewed 500,000-subject DB "
-                                  "(config 4)",
+           "config": {"workload": "batched + learned-cost order: 1,024 queries vs skewed 500,000-subject DB",
                       "queries": len(qs), "subjects": db.num_subjects, "residues": db.num_residues,
                       "db_sha256": db.sha256, "records": int(n2), "groups": groups, "group_residues": gr,
                       "group_queries": gq, "launch": launch,
@@ -386,8 +385,8 @@ def skewed(args):
 def serving(args):
     import threading
     import bpx
-    log("generating config-2 database and the request stream ...")
-    q0, db = config2(args.threads)
+    log("generating the 2,000,000-subject database and the request stream ...")
+    q0, db = plain_workload(args.threads)
     ctx = bpx.Context(0)
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
     rates = [float(x) for x in args.rates.split(",")]
@@ -458,7 +457,7 @@ def serving(args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
            "data": "synthetic (seeded generator); Poisson arrivals",
            "config": {"workload": f"serving: Poisson single queries at {head['rate_qps']:g}/s vs the resident "
-                                  "2,000,000-subject DB (config 5)",
+                                  "2,000,000-subject DB",
                       "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "rates": lines,
                       "query_lengths": "lognormal median 350 sigma 0.6 clipped 20-3000",
                       "checked_against_plain": len(all_results), "mismatches": mismatches,

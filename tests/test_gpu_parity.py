@@ -254,7 +254,7 @@ def test_infeasible_launch_is_refused(ctx):
     assert rc == -5 and lds > ctx.info()["lds_bytes"]
 
 
-def config_db(which):
+def workload_db(which):
     if which == 1:
         q = bpx.synth_query(0, 0, 300)
         spec = dict(seed=0, num_subjects=10_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
@@ -266,22 +266,22 @@ def config_db(which):
     return q, bpx.synth_db(spec, [q], nthreads=THREADS)
 
 
-def test_config1_full(ctx):
-    q, db = config_db(1)
-    say(f"config 1: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
+def test_10k_subject_workload_full(ctx):
+    q, db = workload_db(1)
+    say(f"10,000-subject workload: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
     gpu = gpu_run(ctx, q, db.residues, db.offsets, db.lengths)
     orc = oracle_run(q, db.residues, db.offsets, db.lengths)
     assert_parity(gpu, orc)
     assert len(orc[2]) > 0
 
 
-def test_config2_full(ctx):
-    q, db = config_db(2)
-    say(f"config 2: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
+def test_2M_subject_workload_full(ctx):
+    q, db = workload_db(2)
+    say(f"2,000,000-subject workload: {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256}")
     gpu = gpu_run(ctx, q, db.residues, db.offsets, db.lengths)
-    say(f"config 2: GPU done, {len(gpu[1])} records")
+    say(f"2,000,000-subject workload: GPU done, {len(gpu[1])} records")
     orc = oracle_run(q, db.residues, db.offsets, db.lengths)
-    say(f"config 2: oracle done, {len(orc[2])} records")
+    say(f"2,000,000-subject workload: oracle done, {len(orc[2])} records")
     assert_parity(gpu, orc)
 
 
