@@ -19,7 +19,8 @@
 //   * the ring is drained 64 hits at a time.  Hits that share a diagonal
 //     slot are applied in scan order by claiming a hashed slot with an LDS
 //     atomic-min on the lane number (gpu_blastp_kernel.cu:194-301 applied
-//     exactly, in parallel across slots);
+//     exactly, in parallel across slots); hits left after two claim rounds
+//     return to the front of the ring for the next batch;
 //   * a hit that completes a pair does not extend inline: its slot is marked
 //     pending and the extension is queued.  A later hit on a pending slot is
 //     deferred, in order, and applied again after the queue is flushed, so
@@ -36,6 +37,8 @@
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 namespace bpx {
 
@@ -476,6 +479,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             uint32_t head = 0, count = 0;                 // hit ring (wave-uniform)
             uint32_t ntasks = 0, ndef = 0;                // queued extensions, deferred hits
             uint32_t rep = 0, rep_total = 0;              // deferred hits being re-applied
+            uint32_t carry = 0;                           // ring-front entries left unapplied by the last batch
             int32_t qbase = 0;
 
             // Applies n (<= 64) hits to the diagonal table; lane i holds the
@@ -488,8 +492,22 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // hits strictly in scan order, and a flush is needed only when a
             // queue is full or the subject ends.  The caller guarantees
             // ntasks + n <= kTaskCap and ndef + n <= kDefCap.
-            auto apply = [&](uint32_t e, uint32_t n, bool fresh) {
+            //
+            // `rounds` (a compile-time count) bounds the claim rounds: a ring
+            // batch runs two, and the hits still unapplied -- each lost to a
+            // lower lane on its slot or to a hash collision in both rounds --
+            // go back to the front of the ring, ahead of every newer hit, and
+            // ride in the next batch (lanes below fresh_from of that batch).
+            // A slot's hits keep their order: the lanes left over are later
+            // than every applied hit of their slot and earlier than every hit
+            // behind them in the ring.  Most batches then need two rounds
+            // instead of three or four (chains of hits on one diagonal from
+            // overlapping words are common).  Re-applied deferred hits run
+            // to completion (rounds = 0).
+            auto apply = [&](uint32_t e, uint32_t n, uint32_t fresh_from, auto rounds) {
+                constexpr int kRounds = decltype(rounds)::value;
                 bool active = lane < n;
+                const bool fresh = lane >= fresh_from;
                 const int32_t c = (int32_t)(e >> 16);
                 const int32_t s = qbase + (int32_t)(e & 0xFFFFu);
                 uint32_t j = 0, dbase = q0.dbase, dmask = q0.dmask;
@@ -512,8 +530,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 const uint32_t h2 = ((dslot * 0x9E3779B1u) >> 16) & cmask;
                 bool deferred = false;
                 const int32_t sd = s + doff;
-                for (bool first = true; __ballot(active) != 0; first = false) {
-                    if (!first) h = h2;
+                for (int round = 0; (kRounds == 0 || round < kRounds) && __ballot(active) != 0; ++round) {
+                    if (round > 0) h = h2;
                     if (active) atomicMin(&claim[h], lane);
                     wave_sync();
                     // slot state and claim are read by every lane, together;
@@ -547,6 +565,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         ntasks += (uint32_t)__popcll(tb);
                     }
                     if (write) diag[dslot] = (SlotT)ns;
+                    wave_sync();
+                }
+                if (kRounds > 0) {
+                    // lanes still unapplied go back to the front of the ring,
+                    // in order: they precede every hit still in it
+                    const unsigned long long lb = __ballot(active);
+                    const uint32_t nl = (uint32_t)__popcll(lb);
+                    if (active) ring[(head - nl + rank_below(lb)) & qmask] = e;
+                    head = (head - nl) & qmask;
+                    count += nl;
+                    carry = nl;
                     wave_sync();
                 }
                 const unsigned long long db = __ballot(deferred);
@@ -643,11 +672,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             bool empty_ring = false, full_drain = false;
             for (;;) {
                 uint32_t n, e;
-                bool fresh = false;
                 if (rep < rep_total) {
                     n = min(64u, rep_total - rep);
                     e = lane < n ? dq[rep + lane] : 0u;
                     rep += n;
+                    apply(e, n, 64u, std::integral_constant<int, 0>{});
                 } else {
                     const bool at_end = pos > last;
                     const bool drain_all = at_end || full_drain;
@@ -724,9 +753,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     e = lane < n ? ring[(head + lane) & qmask] : 0u;
                     head = (head + n) & qmask;
                     count -= n;
-                    fresh = true;
+                    apply(e, n, carry, std::integral_constant<int, 2>{});
                 }
-                apply(e, n, fresh);
             }
 
             // ---- per-subject outputs ----
