@@ -125,10 +125,13 @@ __device__ __forceinline__ uint32_t sres(const uint8_t* sbuf, uint32_t smask, co
 // Both X-drop loops run in blocks of kBlock steps: the block's residues and
 // scores are loaded together, then the steps are applied in order in
 // registers under a `done` predicate, so each step sees exactly the
-// sequential running score, maximum and stop test.  RESIDENT (the whole
+// sequential running score, maximum and stop test.  Two steps per block
+// measured fastest: the lanes of a flush end their walks at different
+// steps, and every step a block runs past a lane's end is issued for the
+// whole wave.  RESIDENT (the whole
 // subject is in sbuf): every load is an unconditional LDS read at a clamped
 // index, masked afterwards.
-constexpr int kBlock = 8;
+constexpr int kBlock = 2;
 
 template <bool RESIDENT>
 __device__ __forceinline__ uint32_t sread(const uint8_t* sbuf, uint32_t smask, const uint8_t* S, int32_t p,
