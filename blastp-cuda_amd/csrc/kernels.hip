@@ -15,7 +15,8 @@
 //   * the lanes take 64 subject positions at a time: each forms its word,
 //     tests the presence bits and finds its cell; a DPP prefix sum over the
 //     cell sizes compacts the hits, in (subject offset, query offset) order,
-//     into a per-wave LDS ring;
+//     into a per-wave LDS ring.  In steady state the lookups of the next 64
+//     positions run interleaved with the claim rounds of the current batch;
 //   * the ring is drained 64 hits at a time.  Hits that share a diagonal
 //     slot are applied in scan order by claiming a hashed slot with an LDS
 //     atomic-min on the lane number (gpu_blastp_kernel.cu:194-301 applied
@@ -496,6 +497,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // queue is full or the subject ends.  The caller guarantees
             // ntasks + n <= kTaskCap and ndef + n <= kDefCap.
             //
+            // hook(r) runs right after round r's state reads are issued (r <
+            // rounds; every r is called once, in order, even when the batch
+            // is done early): the next positions' lookups go there, so their
+            // LDS latency overlaps the claim rounds'.
+            //
             // `rounds` (a compile-time count) bounds the claim rounds: a ring
             // batch runs two, and the hits still unapplied -- each lost to a
             // lower lane on its slot or to a hash collision in both rounds --
@@ -507,8 +513,9 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // instead of three or four (chains of hits on one diagonal from
             // overlapping words are common).  Re-applied deferred hits run
             // to completion (rounds = 0).
-            auto apply = [&](uint32_t e, uint32_t n, uint32_t fresh_from, auto rounds) {
+            auto apply = [&](uint32_t e, uint32_t n, uint32_t fresh_from, auto rounds, auto&& hook) {
                 constexpr int kRounds = decltype(rounds)::value;
+                int hooked = 0;
                 bool active = lane < n;
                 const bool fresh = lane >= fresh_from;
                 const int32_t c = (int32_t)(e >> 16);
@@ -541,6 +548,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     // the state machine below is evaluated without branches
                     const uint32_t cl = claim[h];
                     const uint32_t st = diag[dslot];
+                    if (kRounds > 0 && round < kRounds) {
+                        hook(round);
+                        hooked = round + 1;
+                    }
                     wave_sync();
                     const bool win = active && cl == lane;
                     const int32_t v = (int32_t)(st & kVal);
@@ -570,6 +581,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     if (write) diag[dslot] = (SlotT)ns;
                     wave_sync();
                 }
+                if (kRounds > 0)
+                    for (int r = hooked; r < kRounds; ++r) hook(r);
                 if (kRounds > 0) {
                     // lanes still unapplied go back to the front of the ring,
                     // in order: they precede every hit still in it
@@ -673,13 +686,78 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             const int32_t last = slen - WORD;
             int32_t pos = 0;
             bool empty_ring = false, full_drain = false;
+
+            // The lookup of positions pos .. pos + 63 (one per lane) in four
+            // stages, each waiting for the previous one's LDS reads: the
+            // word's residues; its presence bits and rank base; its cell's
+            // start offsets; the hits into the ring.
+            struct Look {
+                uint32_t b[WORD];
+                uint32_t wd, start, end;
+                uint2 gv;
+            };
+            auto look_a = [&](Look& L) {
+                const int32_t s = pos + (int32_t)lane;
+#pragma unroll
+                for (int i = 0; i < WORD; ++i) L.b[i] = sbuf[(s + i) & smask];
+            };
+            auto look_b = [&](Look& L) {
+                uint32_t wd = L.b[0];
+#pragma unroll
+                for (int i = 1; i < WORD; ++i) wd = (wd << 5) | L.b[i];
+                L.wd = wd;
+                L.gv = pvr[wd >> 5];
+            };
+            auto look_c = [&](Look& L) {
+                const uint32_t bit = 1u << (L.wd & 31u);
+                // rank of the word among present words; an absent word reads
+                // the neighbouring (valid) start entries and is masked out
+                const uint32_t r =
+                    min(L.gv.y + (uint32_t)__popc(L.gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
+                if (narrow_starts) {
+                    L.start = starts16[r];
+                    L.end = starts16[r + 1];
+                } else {
+                    L.start = starts32[r];
+                    L.end = starts32[r + 1];
+                }
+            };
+            // appends the hits of as many leading positions as the ring holds
+            auto look_d = [&](Look& L) {
+                const int32_t s = pos + (int32_t)lane;
+                const bool present = (L.gv.x & (1u << (L.wd & 31u))) && s <= last;
+                const uint32_t cnt = present ? L.end - L.start : 0u;
+                const uint32_t incl = wave_inclusive_sum(cnt);
+                const bool fits = incl <= qcap - count;
+                const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
+                if (nfit == 0) {
+                    if (count == 0) {  // a cell larger than the ring: host sizing error
+                        if (lane == 0) atomicOr(p.overflow, 2u);
+                        pos = last + 1;
+                    } else {
+                        empty_ring = true;
+                    }
+                    return;
+                }
+                if (fits && cnt) {
+                    const uint32_t at = head + count + (incl - cnt);
+                    const uint32_t sv = (uint32_t)(s - qbase);
+                    for (uint32_t t = 0; t < cnt; ++t)
+                        ring[(at + t) & qmask] = ((uint32_t)entries[L.start + t] << 16) | sv;
+                }
+                const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
+                count += added;
+                hits += added;
+                pos += (int32_t)nfit;
+                wave_sync();
+            };
             for (;;) {
                 uint32_t n, e;
                 if (rep < rep_total) {
                     n = min(64u, rep_total - rep);
                     e = lane < n ? dq[rep + lane] : 0u;
                     rep += n;
-                    apply(e, n, 64u, std::integral_constant<int, 0>{});
+                    apply(e, n, 64u, std::integral_constant<int, 0>{}, [](int) {});
                 } else {
                     const bool at_end = pos > last;
                     const bool drain_all = at_end || full_drain;
@@ -707,56 +785,31 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                             empty_ring = full_drain = true;
                             continue;
                         }
-                        const int32_t s = pos + (int32_t)lane;
-                        uint32_t wd = sbuf[s & smask];
-#pragma unroll
-                        for (int i = 1; i < WORD; ++i) wd = (wd << 5) | sbuf[(s + i) & smask];
-                        const uint2 gv = pvr[wd >> 5];
-                        const uint32_t bit = 1u << (wd & 31u);
-                        const bool present = (gv.x & bit) && s <= last;
-                        // rank of the word among present words; an absent word
-                        // reads the neighbouring (valid) start entries and is
-                        // masked out
-                        const uint32_t r =
-                            min(gv.y + (uint32_t)__popc(gv.x & (bit - 1u)), g.npresent ? g.npresent - 1u : 0u);
-                        uint32_t start, end;
-                        if (narrow_starts) {
-                            start = starts16[r];
-                            end = starts16[r + 1];
-                        } else {
-                            start = starts32[r];
-                            end = starts32[r + 1];
-                        }
-                        const uint32_t cnt = present ? end - start : 0u;
-                        const uint32_t incl = wave_inclusive_sum(cnt);
-                        const bool fits = incl <= qcap - count;
-                        const uint32_t nfit = (uint32_t)__popcll(__ballot(fits));
-                        if (nfit == 0) {
-                            if (count == 0) {  // a cell larger than the ring: host sizing error
-                                if (lane == 0) atomicOr(p.overflow, 2u);
-                                pos = last + 1;
-                                continue;
-                            }
-                            empty_ring = true;
-                            continue;
-                        }
-                        if (fits && cnt) {
-                            const uint32_t at = head + count + (incl - cnt);
-                            const uint32_t sv = (uint32_t)(s - qbase);
-                            for (uint32_t t = 0; t < cnt; ++t)
-                                ring[(at + t) & qmask] = ((uint32_t)entries[start + t] << 16) | sv;
-                        }
-                        const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
-                        count += added;
-                        hits += added;
-                        pos += (int32_t)nfit;
-                        wave_sync();
+                        Look L;
+                        look_a(L);
+                        look_b(L);
+                        look_c(L);
+                        look_d(L);
                         continue;
                     }
                     e = lane < n ? ring[(head + lane) & qmask] : 0u;
                     head = (head + n) & qmask;
                     count -= n;
-                    apply(e, n, carry, std::integral_constant<int, 2>{});
+                    // a full batch, and the next 64 positions can be looked up
+                    // without a window refill or a rebase: their lookups run
+                    // interleaved with the batch's claim rounds
+                    const bool combo = n == 64 && !drain_all && !empty_ring && pos <= last &&
+                                       (pos + 128 <= whi || whi >= send) && pos + 64 - qbase < 65536;
+                    if (combo) {
+                        Look L;
+                        look_a(L);
+                        apply(e, n, carry, std::integral_constant<int, 2>{}, [&](int stage) {
+                            if (stage == 0) look_b(L); else look_c(L);
+                        });
+                        look_d(L);
+                    } else {
+                        apply(e, n, carry, std::integral_constant<int, 2>{}, [](int) {});
+                    }
                 }
             }
 
