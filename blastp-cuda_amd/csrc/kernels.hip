@@ -797,8 +797,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     count -= n;
                     // a full batch, and the next 64 positions can be looked up
                     // without a window refill or a rebase: their lookups run
-                    // interleaved with the batch's claim rounds
-                    const bool combo = n == 64 && !drain_all && !empty_ring && pos <= last &&
+                    // interleaved with the batch's claim rounds.  Only for a
+                    // table in LDS: an HBM table's lookups would stall the
+                    // rounds (measured 6 % slower on batched scans)
+                    const bool combo = TAB_LDS && n == 64 && !drain_all && !empty_ring && pos <= last &&
                                        (pos + 128 <= whi || whi >= send) && pos + 64 - qbase < 65536;
                     if (combo) {
                         Look L;
