@@ -65,31 +65,40 @@ def batched_workload(threads):
 
 
 def cpu_baseline(q, db, threads, budget_s=20.0):
-    """The C oracle (same semantics, same inputs) on a bounded prefix of the
-    database, timed on this host's cores: median of 3 runs after a warm-up."""
+    """The C oracle (same semantics, same inputs) on bounded prefixes of the
+    database, timed on this host's cores: on `threads` threads, the median of
+    5 runs after a warm-up; on one thread, the median of 3 runs on a smaller
+    prefix after a warm-up."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     lk = O.OracleLookup(q)
-    # size the sample from a short probe so the whole leg stays ~budget_s
-    probe = 2000
-    t = time.perf_counter()
-    O.run_db(q, db.residues, db.offsets[: probe + 1], db.lengths[:probe], nthreads=threads, lookup=lk)
-    dt = max(time.perf_counter() - t, 1e-4)
-    rate = db.offsets[probe] / dt
-    target = int(rate * budget_s / 4)  # 4 timed runs (warm-up + 3)
-    n = int(np.searchsorted(db.offsets, min(target, int(db.offsets[-1]))))
-    n = max(probe, min(n, db.num_subjects))
-    res = int(db.offsets[n])
-    times = []
-    for _ in range(4):
+
+    def timed(nthreads, share, runs):
+        # size the prefix from a short probe so this leg takes ~share of budget_s
+        probe = 2000
         t = time.perf_counter()
-        O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=threads, lookup=lk)
-        times.append(time.perf_counter() - t)
-    med = float(np.median(times[1:]))
+        O.run_db(q, db.residues, db.offsets[: probe + 1], db.lengths[:probe], nthreads=nthreads, lookup=lk)
+        dt = max(time.perf_counter() - t, 1e-4)
+        rate = db.offsets[probe] / dt
+        target = int(rate * budget_s * share / (runs + 1))
+        n = int(np.searchsorted(db.offsets, min(target, int(db.offsets[-1]))))
+        n = max(probe, min(n, db.num_subjects))
+        times = []
+        for _ in range(runs + 1):
+            t = time.perf_counter()
+            O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=nthreads, lookup=lk)
+            times.append(time.perf_counter() - t)
+        res = int(db.offsets[n])
+        return n, res, res / float(np.median(times[1:]))
+
+    n, res, value = timed(threads, 0.6, 5)
+    n1, res1, value1 = timed(1, 0.4, 3)
     lk.close()
-    return {"value": res / med, "unit": "DB residues/s", "cores": threads, "kind": "port",
+    return {"value": value, "unit": "DB residues/s", "cores": threads, "kind": "port",
             "sample": f"first {n} subjects ({res} residues) of the same database and query; "
-                      f"median of 3 runs after a warm-up, {threads} threads"}
+                      f"median of 5 runs after a warm-up, {threads} threads",
+            "value_1thread": value1,
+            "sample_1thread": f"first {n1} subjects ({res1} residues); median of 3 runs after a warm-up, 1 thread"}
 
 
 def main():
