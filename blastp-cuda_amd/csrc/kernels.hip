@@ -457,7 +457,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
         for (;;) {
             uint32_t k = 0;
             if (lane == 0) k = atomicAdd(&ctl[1], 1u);
-            k = uniform(__shfl(k, 0, 64));
+            k = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
             if (k >= w.count) break;
             // descriptors are stored in processing order: consecutive
             // subjects of an item read consecutive entries
@@ -497,10 +497,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // queue is full or the subject ends.  The caller guarantees
             // ntasks + n <= kTaskCap and ndef + n <= kDefCap.
             //
-            // hook(r) runs right after round r's state reads are issued (r <
-            // rounds; every r is called once, in order, even when the batch
-            // is done early): the next positions' lookups go there, so their
-            // LDS latency overlaps the claim rounds'.
+            // hook(r), r = 0, 1, runs right after round r's state reads are
+            // issued (each once, in order; after the rounds when the batch is
+            // done early or has fewer rounds): the next positions' lookups go
+            // there, so their LDS latency overlaps the claim rounds'.
             //
             // `rounds` (a compile-time count) bounds the claim rounds: a ring
             // batch runs two, and the hits still unapplied -- each lost to a
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     wave_sync();
                 }
                 if (kRounds > 0)
-                    for (int r = hooked; r < kRounds; ++r) hook(r);
+                    for (int r = hooked; r < 2; ++r) hook(r);
                 if (kRounds > 0) {
                     // lanes still unapplied go back to the front of the ring,
                     // in order: they precede every hit still in it
@@ -745,7 +745,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     for (uint32_t t = 0; t < cnt; ++t)
                         ring[(at + t) & qmask] = ((uint32_t)entries[L.start + t] << 16) | sv;
                 }
-                const uint32_t added = uniform(__shfl(incl, (int)nfit - 1, 64));
+                // nfit is wave-uniform: a lane read, not an LDS permute
+                const uint32_t added = (uint32_t)__builtin_amdgcn_readlane((int)incl, (int)nfit - 1);
                 count += added;
                 hits += added;
                 pos += (int32_t)nfit;
