@@ -155,6 +155,8 @@ _sig("bpx_volume_write", C.c_int, C.c_char_p, C.c_char_p, _vp, _vp, _vp, C.c_uin
 _sig("bpx_db_volumes", C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32))
 _sig("bpx_db_load_volumes", C.c_int, _vp, C.c_char_p, C.POINTER(_vp))
 _sig("bpx_volume_last_error", C.c_char_p)
+_sig("bpx_gpu_info", C.c_int, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32))
+_sig("bpx_gpu_volume_read", C.c_int, C.c_char_p, C.c_uint32, C.POINTER(_vp))
 _sig("bpx_db_features", C.c_int, _vp, _vp, _vp, C.c_int32, _vp)
 _sig("bpx_db_set_cost", C.c_int, _vp, _vp)
 _sig("bpx_costmodel_create", C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp))
@@ -717,10 +719,7 @@ def volume_write(base: str, residues, offsets, lengths, title: str = ""):
     _vchk(_lib.bpx_volume_write(base.encode(), title.encode(), _ptr(r), _ptr(o), _ptr(ln), len(ln)))
 
 
-def volume_read(base: str):
-    """Reads base.pin / base.psq: (residues, offsets[n+1], lengths[n], title)."""
-    h = _vp()
-    _vchk(_lib.bpx_volume_read(base.encode(), C.byref(h)))
+def _volume_arrays(h):
     try:
         n, tot, mx, title = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_char_p()
         _vchk(_lib.bpx_volume_info(h, C.byref(n), C.byref(tot), C.byref(mx), C.byref(title)))
@@ -736,6 +735,28 @@ def volume_read(base: str):
         return arr(rp, tot, np.uint8), arr(op, n + 1, np.uint64), arr(lp, n, np.uint32), title.value.decode()
     finally:
         _lib.bpx_volume_destroy(h)
+
+
+def volume_read(base: str):
+    """Reads base.pin / base.psq: (residues, offsets[n+1], lengths[n], title)."""
+    h = _vp()
+    _vchk(_lib.bpx_volume_read(base.encode(), C.byref(h)))
+    return _volume_arrays(h)
+
+
+def gpu_info(gpuinfo: str):
+    """The header of a .gpuinfo file: (volumes, group size, percentage)."""
+    nv, gs, pc = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _vchk(_lib.bpx_gpu_info(gpuinfo.encode(), C.byref(nv), C.byref(gs), C.byref(pc)))
+    return int(nv.value), int(gs.value), int(pc.value)
+
+
+def gpu_volume_read(gpuinfo: str, volume: int = 0):
+    """Volume `volume` of a .gpu database, de-interleaved:
+    (residues, offsets[n+1], lengths[n], "")."""
+    h = _vp()
+    _vchk(_lib.bpx_gpu_volume_read(gpuinfo.encode(), volume, C.byref(h)))
+    return _volume_arrays(h)
 
 
 def db_volumes(base: str):

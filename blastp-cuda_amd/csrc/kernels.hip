@@ -83,10 +83,6 @@ __device__ __forceinline__ uint32_t rank_below(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-__device__ __forceinline__ uint32_t uniform(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-
 // Inclusive prefix sum over the 64 lanes with DPP: Hillis-Steele inside each
 // 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's
 // into row 3 (row_bcast:15), then lane 31's total into rows 2-3

@@ -17,6 +17,7 @@
 // headers), for tests and for packing synthetic databases.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -315,6 +316,165 @@ int bpx_db_load_volumes(bpx_ctx* ctx, const char* base, bpx_db** out) {
     }
     if (len.size() > 0xFFFFFFFFull) return vfail(BPX_E_ARG, "more than 2^32 sequences");
     return bpx_db_load(ctx, res.data(), off.data(), len.data(), (uint32_t)len.size(), out);
+}
+
+// ---- the reference's own GPU database files (.gpu / .gpuinfo) -------------
+//
+// .gpuinfo is text (Write_GPU_information_file_preliminary / _file,
+// functions.cpp:1-46): a comment line before every value; blocks, threads,
+// percentage, number of volumes; then per volume its .gpu file name, byte
+// size, sequences in the volume, sequences in the .gpu file, store limit,
+// break limit, number of groups and each group's length.  A .gpu file
+// (CreateGPU_BLASTP_database, functions.cpp:364-613) holds the first
+// sequences of the volume (ascending length) in groups of G = blocks *
+// threads: group g spans len_g * G bytes (len_g: its longest sequence rounded
+// up to 64), and residue 4i + j of the group's m-th sequence sits at byte
+// (m + i * G) * 4 + j of the group; the rest is zero.  The files hold no
+// per-sequence length: protein residues are never 0, so a sequence ends at
+// its last non-zero byte.
+
+namespace {
+
+struct GpuInfo {
+    int64_t blocks = 0, threads = 0, percentage = 0;
+    struct Vol {
+        std::string name;
+        uint64_t bytes = 0;
+        int64_t volume_length = 0, num_sequences = 0, store_limit = 0, break_limit = 0;
+        std::vector<int64_t> group_len;
+    };
+    std::vector<Vol> vols;
+};
+
+int parse_gpuinfo(const std::string& path, GpuInfo* gi) {
+    std::vector<uint8_t> raw;
+    if (!read_file(path, &raw)) return vfail(BPX_E_ARG, "cannot read " + path);
+    std::vector<std::string> vals;  // every non-comment line, in order
+    std::string text(raw.begin(), raw.end());
+    size_t at = 0;
+    while (at < text.size()) {
+        size_t eol = text.find('\n', at);
+        if (eol == std::string::npos) eol = text.size();
+        std::string line = text.substr(at, eol - at);
+        at = eol + 1;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (!line.empty() && line[0] == '#') continue;
+        vals.push_back(line);
+    }
+    size_t k = 0;
+    bool ok = true;
+    auto num = [&](int64_t* v) {
+        if (k >= vals.size()) {
+            ok = false;
+            return;
+        }
+        char* end = nullptr;
+        const long long x = std::strtoll(vals[k].c_str(), &end, 10);
+        if (end == vals[k].c_str()) ok = false;
+        *v = x;
+        ++k;
+    };
+    int64_t nv = 0;
+    num(&gi->blocks);
+    num(&gi->threads);
+    num(&gi->percentage);
+    num(&nv);
+    if (!ok || gi->blocks <= 0 || gi->threads <= 0 || nv < 0 || nv > 1000000)
+        return vfail(BPX_E_ARG, path + ": malformed header (blocks, threads, percentage, volumes)");
+    for (int64_t v = 0; v < nv && ok; ++v) {
+        GpuInfo::Vol vol;
+        if (k >= vals.size()) {
+            ok = false;
+            break;
+        }
+        vol.name = vals[k++];
+        int64_t bytes = 0, groups = 0;
+        num(&bytes);
+        num(&vol.volume_length);
+        num(&vol.num_sequences);
+        num(&vol.store_limit);
+        num(&vol.break_limit);
+        num(&groups);
+        if (!ok || bytes < 0 || groups < 0 || groups > (1 << 30) || vol.num_sequences < 0) {
+            ok = false;
+            break;
+        }
+        vol.bytes = (uint64_t)bytes;
+        vol.group_len.resize((size_t)groups);
+        for (auto& g : vol.group_len) num(&g);
+        gi->vols.push_back(std::move(vol));
+    }
+    if (!ok) return vfail(BPX_E_ARG, path + ": malformed volume record");
+    return BPX_OK;
+}
+
+}  // namespace
+
+int bpx_gpu_info(const char* gpuinfo, uint32_t* num_volumes, uint32_t* group_size, uint32_t* percentage) {
+    if (!gpuinfo) return vfail(BPX_E_ARG, "NULL argument");
+    GpuInfo gi;
+    const int rc = parse_gpuinfo(gpuinfo, &gi);
+    if (rc) return rc;
+    if (num_volumes) *num_volumes = (uint32_t)gi.vols.size();
+    if (group_size) *group_size = (uint32_t)(gi.blocks * gi.threads);
+    if (percentage) *percentage = (uint32_t)gi.percentage;
+    return BPX_OK;
+}
+
+int bpx_gpu_volume_read(const char* gpuinfo, uint32_t volume, bpx_volume** out) {
+    if (!gpuinfo || !out) return vfail(BPX_E_ARG, "NULL argument");
+    *out = nullptr;
+    GpuInfo gi;
+    int rc = parse_gpuinfo(gpuinfo, &gi);
+    if (rc) return rc;
+    if (volume >= gi.vols.size()) return vfail(BPX_E_ARG, "volume index out of range");
+    const GpuInfo::Vol& vol = gi.vols[volume];
+    const uint64_t G = (uint64_t)(gi.blocks * gi.threads);
+    const uint64_t n = (uint64_t)vol.num_sequences;
+    const uint64_t ngroups = vol.group_len.size();
+    if (ngroups != (n + G - 1) / G) return vfail(BPX_E_ARG, "group count does not match the sequence count");
+    uint64_t bytes = 0;
+    for (int64_t L : vol.group_len) {
+        if (L < 0 || L > (1 << 30)) return vfail(BPX_E_ARG, "bad group length");
+        bytes += (uint64_t)L * G;
+    }
+    if (bytes != vol.bytes) return vfail(BPX_E_ARG, "volume size does not match its group lengths");
+    // the name as written; else the same file name next to the .gpuinfo
+    std::vector<uint8_t> db;
+    std::string name = vol.name;
+    if (!read_file(name, &db)) {
+        const std::string info(gpuinfo);
+        const std::string dir = info.find('/') == std::string::npos ? "" : info.substr(0, info.rfind('/') + 1);
+        const std::string leaf = name.find('/') == std::string::npos ? name : name.substr(name.rfind('/') + 1);
+        name = dir + leaf;
+        if (!read_file(name, &db)) return vfail(BPX_E_ARG, "cannot read " + vol.name);
+    }
+    if (db.size() != bytes) return vfail(BPX_E_ARG, name + ": size differs from the .gpuinfo record");
+    auto* v = new (std::nothrow) bpx_volume;
+    if (!v) return vfail(BPX_E_NOMEM, "out of memory");
+    v->version = 0;
+    v->n = (uint32_t)n;
+    v->offsets.reserve(n + 1);
+    v->lengths.reserve(n);
+    uint64_t gbase = 0;
+    for (uint64_t g = 0; g < ngroups; ++g) {
+        const uint64_t L = (uint64_t)vol.group_len[g];
+        const uint64_t members = std::min<uint64_t>(G, n - g * G);
+        for (uint64_t m = 0; m < members; ++m) {
+            auto at = [&](uint64_t r) { return gbase + (m + (r / 4) * G) * 4 + (r & 3); };
+            uint64_t len = L;
+            while (len > 0 && db[at(len - 1)] == 0) --len;
+            v->offsets.push_back(v->residues.size());
+            v->lengths.push_back((uint32_t)len);
+            for (uint64_t r = 0; r < len; ++r) v->residues.push_back(db[at(r)]);
+            v->max_len = std::max<uint32_t>(v->max_len, (uint32_t)len);
+        }
+        gbase += L * G;
+    }
+    v->offsets.push_back(v->residues.size());
+    v->total = v->residues.size();
+    *out = v;
+    return BPX_OK;
 }
 
 }  // extern "C"

@@ -347,6 +347,20 @@ int bpx_db_volumes(const char *base, char *names, size_t capacity, size_t *neede
 int bpx_db_load_volumes(bpx_ctx *ctx, const char *base, bpx_db **out);
 const char *bpx_volume_last_error(void);
 
+/* ---- the reference's own GPU database files (.gpu / .gpuinfo) ------------
+ * Replaces ReadGPU_BLASTP_database + Read_GPU_information_file(_preliminary)
+ * (functions.cpp:47-179, 618-639), which read the files that
+ * CreateGPU_BLASTP_database (functions.cpp:364-613) writes: sequences in
+ * length order, interleaved 4 residues at a time within groups of
+ * blocks * threads sequences.  bpx_gpu_info reads the .gpuinfo header;
+ * bpx_gpu_volume_read de-interleaves volume `volume` into a bpx_volume
+ * (sequence i = OID break-limit-of-the-previous-volume + i; lengths from the
+ * last non-zero residue, since the files hold none).  The .gpu file is
+ * opened by the name the .gpuinfo records, else by that file name next to
+ * the .gpuinfo.  Host-only. */
+int bpx_gpu_info(const char *gpuinfo, uint32_t *num_volumes, uint32_t *group_size, uint32_t *percentage);
+int bpx_gpu_volume_read(const char *gpuinfo, uint32_t volume, bpx_volume **out);
+
 /* The built-in BLOSUM62 table (784 bytes, ncbistdaa order). */
 const int8_t *bpx_blosum62(void);
 
