@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--workload", choices=("plain", "batched", "skewed", "serving"), default="plain")
-    ap.add_argument("--rates", default="25,50,200", help="serving: arrival rates (queries/s), comma-separated")
+    ap.add_argument("--rates", default="25,50,200,2000", help="serving: arrival rates (queries/s), comma-separated")
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per arrival rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a plain elementwise copy (torch.neg over int32) reading and writing BYTES, "
