@@ -28,7 +28,7 @@
 //     every slot still sees its hits strictly in scan order.  Queued
 //     extensions (s_BlastAaExtendTwoHit/Left/Right semantics,
 //     gpu_blastp_kernel.cu:341-549) run when a queue fills or the subject
-//     ends: one per lane, or a few on 16-lane rows;
+//     ends: one per lane, or a few on 8-lane segments;
 //   * records go to the wave's private output segment; per-(query, subject)
 //     counts, optional counters and per-query ungapped statistics are written
 //     when the subject is done.
