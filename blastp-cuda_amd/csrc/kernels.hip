@@ -701,6 +701,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 uint32_t wd = L.b[0];
 #pragma unroll
                 for (int i = 1; i < WORD; ++i) wd = (wd << 5) | L.b[i];
+                // lanes past the subject's end read buffer bytes that may
+                // never have been written (any value): keep their word in
+                // range so its presence-word read stays inside the table
+                wd &= (1u << (5 * WORD)) - 1u;
                 L.wd = wd;
                 L.gv = pvr[wd >> 5];
             };
