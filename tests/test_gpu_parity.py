@@ -243,6 +243,37 @@ def test_launch_shapes_do_not_change_results(ctx, launch):
     assert_parity(gpu_run(ctx, q, res, offs, lens, launch=launch), oracle_run(q, res, offs, lens))
 
 
+@pytest.mark.parametrize("launch", [None, {"waves_per_group": 1}, {"subject_buffer": 1024}])
+def test_low_complexity_chains_and_queue_caps(ctx, launch):
+    """Homopolymer runs and tandem repeats against a repetitive query: long
+    chains of hits on one diagonal (more than two claim rounds, so hits ride
+    over to the next batch), cells larger than a batch, and more pairs and
+    deferred hits than the per-wave queues hold (flushes in mid-subject)."""
+    rng = np.random.default_rng(91)
+    motif = rng.choice(STD20, 7)
+    q = np.concatenate([np.tile(motif, 30), np.full(60, STD20[3], np.uint8), rng.choice(STD20, 90)])
+    subjects = []
+    for i in range(400):
+        kind = i % 4
+        n = int(rng.integers(40, 2600))
+        if kind == 0:
+            s = np.full(n, STD20[3], np.uint8)                       # homopolymer
+        elif kind == 1:
+            s = np.tile(motif, n // 7 + 1)[:n].copy()                # tandem repeat of the query's motif
+        elif kind == 2:
+            s = np.tile(rng.choice(STD20, int(rng.integers(2, 6))), n // 2 + 1)[:n].copy()
+        else:
+            s = rng.choice(STD20, n)
+            at = int(rng.integers(0, max(1, n - 200)))
+            s[at:at + 200] = np.tile(motif, 29)[: len(s[at:at + 200])]
+        mut = rng.random(n) < 0.05
+        s[mut] = rng.choice(STD20, int(mut.sum()))
+        subjects.append(s)
+    res, offs, lens = pack(subjects)
+    say(f"low complexity, launch {launch}: {int(lens.sum())} residues")
+    assert_parity(gpu_run(ctx, q, res, offs, lens, launch=launch), oracle_run(q, res, offs, lens))
+
+
 def test_infeasible_launch_is_refused(ctx):
     q = bpx.synth_query(3, 1, 350)
     res, offs, lens = pack([q, q])
