@@ -383,10 +383,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t* claim = reinterpret_cast<uint32_t*>(wbase + p.sbuf + diag_bytes);
     uint32_t* ring = claim + p.claim_slots;
     // queued extension: {slot | second hit's group position << 16, subject
-    // offset}; the first hit stays in the pending slot's value bits
+    // offset - qbase (16 bits: qbase does not move while extensions are
+    // queued)}; the first hit stays in the pending slot's value bits
     uint32_t* t_sc = ring + p.qcap;
-    uint32_t* t_s = t_sc + kTaskCap;
-    uint32_t* dq = t_s + kTaskCap;
+    uint16_t* t_s = reinterpret_cast<uint16_t*>(t_sc + kTaskCap);  // subject offset - qbase
+    uint32_t* dq = reinterpret_cast<uint32_t*>(t_s + kTaskCap);
     QCount* qc = reinterpret_cast<QCount*>(dq + kDefCap);
     // zero-fills the wave's diagonal table, 16 bytes per lane and store
     auto clear_diag = [&]() {
@@ -570,7 +571,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         if (task) {
                             const uint32_t idx = ntasks + rank_below(tb);
                             t_sc[idx] = dslot | ((uint32_t)c << 16);
-                            t_s[idx] = (uint32_t)s;
+                            t_s[idx] = (uint16_t)(s - qbase);
                         }
                         ntasks += (uint32_t)__popcll(tb);
                     }
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     const uint32_t tslot = sc & 0xFFFFu;
                     int32_t tc = (int32_t)(sc >> 16);
                     const uint32_t tj = MULTI ? (uint32_t)qmap[tc] : 0u;
-                    const int32_t ts = (int32_t)t_s[ti];
+                    const int32_t ts = qbase + (int32_t)t_s[ti];
                     // left boundary: the first hit (kept in the slot) + word
                     const int32_t tsl = (int32_t)(diag[tslot] & kVal) - doff + WORD;
                     const QueryInfo qi = MULTI ? qinfo[tj] : q0;
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // range so its presence-word read stays inside the table
                 wd &= (1u << (5 * WORD)) - 1u;
                 L.wd = wd;
-                L.gv = pvr[wd >> 5];
+                L.gv = pvr[min(wd >> 5, p.ngroups_pv - 1u)];
             };
             auto look_c = [&](Look& L) {
                 const uint32_t bit = 1u << (L.wd & 31u);
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                             whi = nhi;
                             wave_sync();
                         }
-                        if (count == 0 && ndef == 0) {
+                        if (count == 0 && ndef == 0 && ntasks == 0) {
                             qbase = pos;
                         } else if (pos + 64 - qbase >= 65536) {
                             empty_ring = full_drain = true;
@@ -1042,7 +1043,7 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     pl.qcap = qcap;
     pl.claim_slots = claim_slots;
     pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
-    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + claim_slots * 4 + qcap * 4 + kTaskCap * 8 + kDefCap * 4 +
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + claim_slots * 4 + qcap * 4 + kTaskCap * 6 + kDefCap * 4 +
                      (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
     uint32_t at = kCtlBytes;
     pl.table = at;

@@ -258,7 +258,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
     set.nq = (uint32_t)qs.size();
     set.word = word;
     set.window = window;
-    set.ngroups_pv = (1u << (5 * word)) / 32;
+    set.ngroups_pv = bpx::packed_presence_groups(word);
     // Pack short queries (ascending length) into groups; long ones alone.
     std::vector<uint32_t> order(qs.size());
     std::iota(order.begin(), order.end(), 0u);

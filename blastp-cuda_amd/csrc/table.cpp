@@ -170,17 +170,19 @@ bool build_table_multi(const QuerySpan* queries, int32_t nq, const int8_t* M,
 
 bool HostTable::narrow_starts() const { return entries.size() <= 0xFFFFu; }
 
+uint32_t HostTable::packed_groups() const { return packed_presence_groups(word_length); }
+
 uint64_t HostTable::packed_bytes() const {
     const uint64_t e = ((uint64_t)entries.size() * 2 + 3) & ~3ull;
     const uint64_t st = narrow_starts() ? (2ull * (npresent + 1) + 3) & ~3ull : 4ull * (npresent + 1);
-    return 8ull * ngroups + st + e;
+    return 8ull * packed_groups() + st + e;
 }
 
 std::vector<uint32_t> HostTable::pack() const {
     std::vector<uint32_t> blob(packed_bytes() / 4, 0);
     uint8_t* b = reinterpret_cast<uint8_t*>(blob.data());
-    std::memcpy(b, groups.data(), 8ull * ngroups);
-    b += 8ull * ngroups;
+    std::memcpy(b, groups.data(), 8ull * packed_groups());
+    b += 8ull * packed_groups();
     if (narrow_starts()) {
         uint16_t* s16 = reinterpret_cast<uint16_t*>(b);
         for (uint32_t r = 0; r <= npresent; ++r) s16[r] = (uint16_t)starts[r];

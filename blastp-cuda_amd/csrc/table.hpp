@@ -25,6 +25,12 @@ namespace bpx {
 constexpr int kAlphabet = 28;
 constexpr int kCharsize = 5;
 
+// Presence words a packed table keeps for words of word_length residues:
+// those covering codes whose first residue is < 28 (of the 5-bit codes).
+inline uint32_t packed_presence_groups(int32_t word_length) {
+    return ((28u << (5 * (word_length - 1))) + 31u) / 32u;
+}
+
 struct HostTable {
     int32_t word_length = 0;
     uint32_t nwords = 0;                // 1 << (5 * word_length)
@@ -40,6 +46,9 @@ struct HostTable {
     // starts are u16 (padded to 4 B) when the table has at most 65,535
     // entries and u32 otherwise.
     bool narrow_starts() const;
+    // Presence words kept in the packed table: only words whose first
+    // residue code is < 28 can occur, so the rest (always empty) are dropped.
+    uint32_t packed_groups() const;
     std::vector<uint32_t> pack() const;
     uint64_t packed_bytes() const;
 };
