@@ -88,7 +88,7 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         return BPX_OK;
     };
 
-    static const int32_t kPolicies[][2] = {{600, 32}, {1200, 64}, {2400, 128}, {4000, 255}};
+    static const int32_t kPolicies[][2] = {{600, 32}, {1200, 64}, {2000, 128}, {2400, 128}, {4000, 255}};
     for (const auto& pq : kPolicies) {
         Candidate c{};
         c.policy.group_residues = pq[0];
@@ -103,7 +103,7 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         if ((rc = consider(c)) != BPX_OK) return rc;
     }
     const Candidate base2 = best;
-    for (int32_t items : {256, 1024}) {
+    for (int32_t items : {256, 512, 1024}) {
         Candidate c = base2;
         c.launch.subjects_per_item = items;
         if ((rc = consider(c)) != BPX_OK) return rc;

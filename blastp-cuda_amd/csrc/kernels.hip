@@ -470,6 +470,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 for (int32_t x = from + 16 * (int32_t)lane; x < to; x += 1024)
                     *reinterpret_cast<uint4*>(sbuf + (x & smask)) = *reinterpret_cast<const uint4*>(S + x);
             };
+            if (NARROW && doff + slen + window > (int32_t)kVal) {
+                clear_diag();
+                doff = window;
+            }
             const bool resident = send <= (int32_t)p.sbuf;
             int32_t whi = min(send, (int32_t)p.sbuf);
             fill(0, whi);
@@ -873,9 +877,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 wave_sync();
             }
             // Blast_ExtendWordExit: a fresh diagonal range for the next
-            // subject.  Narrow slots: clear the table; wide slots: advance
-            // the running offset, clearing the table before it runs out.
-            if (NARROW || doff >= kOffsetLimit) {
+            // subject -- NCBI's running offset, advanced by the subject's
+            // length + window, so every slot a previous subject left reads as
+            // a hit at least a window back (or a flag behind the next hit).
+            // The table is cleared only when the offset would outgrow the
+            // slot's value bits: narrow slots check at the next subject's
+            // start (its length decides), wide slots here.
+            if (!NARROW && doff >= kOffsetLimit) {
                 clear_diag();
                 doff = window;
                 wave_sync();
