@@ -408,7 +408,10 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
         }
         return pl;
     };
-    static const int kAutoWaves[] = {16, 12, 8, 4, 2, 1};
+    // every wave count from 16 down to 8: a packed group's per-wave
+    // diagonal tables often fit 13-15 waves (measured 2 % faster on the
+    // 1,024-query batch than rounding down to 12)
+    static const int kAutoWaves[] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 6, 4, 2, 1};
     const int* cands = auto_waves ? kAutoWaves : &user_waves;
     const int ncands = auto_waves ? (int)(sizeof kAutoWaves / sizeof kAutoWaves[0]) : 1;
     // the most waves (of the candidates) at which `a` fits for every slot
