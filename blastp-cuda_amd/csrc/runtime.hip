@@ -48,7 +48,7 @@ int fail(int code, const char* fmt, ...) {
 
 constexpr uint32_t kArenaAlign = 128;  // subject starts in the arena: whole cache lines per subject
 constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last subject
-constexpr int kMaxClasses = 64;        // launches per scan: (table placement, multi, waves) bins x slot width
+constexpr int kMaxClasses = 128;       // launches per scan: (table placement, multi, waves) bins x slot width (<= 2 x 2 x 13 x 2)
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 constexpr uint32_t kSegFloor = 1024;   // smallest per-wave record segment (64 B per record slot)
 
