@@ -351,9 +351,12 @@ __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uin
 // Per-query counters of a wave (MULTI groups), 32 bytes each.
 struct QCount {
     uint32_t hits, ext, saved, pad;
-    unsigned long long agg_hits;
-    uint32_t agg_nseq, agg_ext, agg_saved, agg_passed;
 };
+// Per-query ungapped statistics of a work item (MULTI groups) live in the
+// registers of the lane that owns the query (lane j owns queries j, j + 64,
+// j + 128, j + 192), not in LDS: a smaller per-wave LDS share leaves room
+// for more waves.
+constexpr int kQuerySlots = 4;  // queries per lane: groups hold at most 255
 
 template <int WORD, bool TAB_LDS, bool MULTI, bool NARROW>
 __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
@@ -449,6 +452,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                                                 : reinterpret_cast<const uint16_t*>(starts32 + g.npresent + 1);
         const QueryInfo q0 = qinfo[0];
         uint32_t a_nseq = 0, a_ext = 0, a_saved = 0, a_passed = 0;  // single-query aggregates
+        unsigned long long m_hits[kQuerySlots];                        // MULTI: per owned query
+        uint32_t m_nseq[kQuerySlots], m_ext[kQuerySlots], m_saved[kQuerySlots], m_passed[kQuerySlots];
+#pragma unroll
+        for (int t = 0; t < kQuerySlots; ++t) {
+            m_hits[t] = 0;
+            m_nseq[t] = m_ext[t] = m_saved[t] = m_passed[t] = 0;
+        }
         unsigned long long a_hits = 0;
 
         for (;;) {
@@ -851,24 +861,28 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 }
             } else {
                 wave_sync();
-                for (uint32_t jj = lane; jj < g.nq; jj += 64) {
-                    QCount& cq = qc[jj];
-                    const size_t qrow = (size_t)qinfo[jj].qid * p.count;
-                    if (cq.saved) p.cnt[qrow + slot] = cq.saved;
-                    if (p.stats) {
-                        int32_t* st = p.stats + 3 * (qrow + opos);
-                        st[0] = (int32_t)cq.hits;
-                        st[1] = (int32_t)cq.saved;
-                        st[2] = (int32_t)cq.ext;
+#pragma unroll
+                for (int t = 0; t < kQuerySlots; ++t) {
+                    const uint32_t jj = lane + 64u * (uint32_t)t;
+                    if (jj < g.nq) {
+                        QCount& cq = qc[jj];
+                        const size_t qrow = (size_t)qinfo[jj].qid * p.count;
+                        if (cq.saved) p.cnt[qrow + slot] = cq.saved;
+                        if (p.stats) {
+                            int32_t* st = p.stats + 3 * (qrow + opos);
+                            st[0] = (int32_t)cq.hits;
+                            st[1] = (int32_t)cq.saved;
+                            st[2] = (int32_t)cq.ext;
+                        }
+                        if (cq.hits > 0) {
+                            m_hits[t] += cq.hits;
+                            m_nseq[t] += 1;
+                            m_ext[t] += cq.ext;
+                            m_saved[t] += cq.saved;
+                            m_passed[t] += cq.saved > 0;
+                        }
+                        cq.hits = cq.ext = cq.saved = 0;
                     }
-                    if (cq.hits > 0) {
-                        cq.agg_hits += cq.hits;
-                        cq.agg_nseq += 1;
-                        cq.agg_ext += cq.ext;
-                        cq.agg_saved += cq.saved;
-                        cq.agg_passed += cq.saved > 0;
-                    }
-                    cq.hits = cq.ext = cq.saved = 0;
                 }
                 if (lane == 0 && has_run) {
                     p.run_loc[gslot] = seg_base + rec_start;
@@ -903,17 +917,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 atomicAdd(&u[4], (unsigned long long)a_passed);
             }
         } else {
-            for (uint32_t jj = lane; jj < g.nq; jj += 64) {
-                QCount& cq = qc[jj];
-                if (cq.agg_nseq) {
+#pragma unroll
+            for (int t = 0; t < kQuerySlots; ++t) {
+                const uint32_t jj = lane + 64u * (uint32_t)t;
+                if (jj < g.nq && m_nseq[t]) {
                     unsigned long long* u = p.ustats + 5 * (size_t)qinfo[jj].qid;
-                    atomicAdd(&u[0], cq.agg_hits);
-                    atomicAdd(&u[1], (unsigned long long)cq.agg_nseq);
-                    atomicAdd(&u[2], (unsigned long long)cq.agg_ext);
-                    atomicAdd(&u[3], (unsigned long long)cq.agg_saved);
-                    atomicAdd(&u[4], (unsigned long long)cq.agg_passed);
+                    atomicAdd(&u[0], m_hits[t]);
+                    atomicAdd(&u[1], (unsigned long long)m_nseq[t]);
+                    atomicAdd(&u[2], (unsigned long long)m_ext[t]);
+                    atomicAdd(&u[3], (unsigned long long)m_saved[t]);
+                    atomicAdd(&u[4], (unsigned long long)m_passed[t]);
                 }
-                cq = QCount{};
             }
         }
     }
