@@ -242,9 +242,9 @@ def batched(args):
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
     work = db.num_residues * len(qs)  # DB residues summed over queries
 
-    # autotune on a 5 % sample (subjects [0, N/20))
+    # autotune on a 10 % sample (subjects [0, N/10))
     t = time.perf_counter()
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 20)
+    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
     log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
         f"{len(trials)} trials)")
     for tr in trials:
