@@ -340,7 +340,7 @@ def skewed(args):
     train_s = time.perf_counter() - t
     log(f"  cost model: trained in {train_s:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}")
 
-    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 20)
+    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
     b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
     groups = b.info()["groups"]
     log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch}; {groups} groups")
