@@ -833,16 +833,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
 
             // ---- per-subject outputs ----
             // Counts and run lengths were zero-filled before the launch, so
-            // only subjects with records write them; counters go to their
-            // processing-order position (sequential within an item) and are
-            // put in subject order after the scan.
+            // only subjects with records write them.  The per-subject
+            // counters go straight to pinned host memory at the subject's
+            // slot: 12-byte posted writes over the host link while the scan
+            // runs, instead of a permute and a copy after it.
             const uint32_t gslot = w.group * p.count + slot;
             const bool has_run = seg_fill > rec_start;
             if (!MULTI) {
                 if (lane == 0) {
                     if (saved) p.cnt[(size_t)q0.qid * p.count + slot] = saved;
                     if (p.stats) {
-                        int32_t* st = p.stats + 3 * ((size_t)q0.qid * p.count + opos);
+                        int32_t* st = p.stats + 3 * ((size_t)q0.qid * p.count + slot);
                         st[0] = (int32_t)hits;
                         st[1] = (int32_t)saved;
                         st[2] = (int32_t)extended;
@@ -869,7 +870,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         const size_t qrow = (size_t)qinfo[jj].qid * p.count;
                         if (cq.saved) p.cnt[qrow + slot] = cq.saved;
                         if (p.stats) {
-                            int32_t* st = p.stats + 3 * (qrow + opos);
+                            int32_t* st = p.stats + 3 * (qrow + slot);
                             st[0] = (int32_t)cq.hits;
                             st[1] = (int32_t)cq.saved;
                             st[2] = (int32_t)cq.ext;
@@ -1038,18 +1039,6 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
     }
 }
 
-// per-(query, position) counters -> per-(query, subject) order
-__global__ void k_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
-                                uint32_t count, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t q = i / count, pos = i % count;
-        const uint64_t j = q * count + (order[pos] - first);
-        dst[3 * j + 0] = src[3 * i + 0];
-        dst[3 * j + 1] = src[3 * i + 1];
-        dst[3 * j + 2] = src[3 * i + 2];
-    }
-}
-
 __global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = value;
@@ -1139,14 +1128,6 @@ hipError_t launch_compact(const CompactParams& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
-                                uint32_t count, uint64_t n, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    const uint64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_permute_stats, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, stream, src, dst,
-                       order, first, count, n);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;

@@ -78,7 +78,7 @@ struct ScanParams {
     uint32_t* work_counter;
     // outputs
     uint32_t* cnt;              // saved records per (qid, slot): qid * count + slot; zero-filled
-    int32_t* stats;             // optional: 3 per (qid, processing position): qid * count + k
+    int32_t* stats;             // optional: 3 per (qid, slot): qid * count + slot, in pinned host memory
     uint32_t* run_loc;          // per (group, slot): first segment record of the run
     uint32_t* run_len;          // per (group, slot): records in the run; zero-filled
     unsigned long long* ustats; // 5 per qid (bpx_ungapped_stats order)
@@ -150,8 +150,5 @@ constexpr uint32_t kTile = 1024;
 hipError_t launch_compact(const CompactParams& p, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream);
-// counters per (query, processing position) -> per (query, subject slot)
-hipError_t launch_permute_stats(const int32_t* src, int32_t* dst, const uint32_t* order, uint32_t first,
-                                uint32_t count, uint64_t n, hipStream_t stream);
 
 }  // namespace bpx

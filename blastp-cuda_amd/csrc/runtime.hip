@@ -53,10 +53,12 @@ constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 constexpr uint32_t kSegFloor = 1024;   // smallest per-wave record segment (64 B per record slot)
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
-// do not call hipMalloc/hipFree.
+// do not call hipMalloc/hipFree.  Host pools pass their hipHostMalloc flags.
 struct Pool {
-    explicit Pool(bool host = false) : host_(host) {}
-    bool host_;
+    Pool() = default;
+    explicit Pool(unsigned host_flags) : host_(true), flags_(host_flags) {}
+    bool host_ = false;
+    unsigned flags_ = 0;
     std::multimap<size_t, void*> free_;
     std::map<void*, size_t> sizes_;
     ~Pool() {
@@ -69,7 +71,7 @@ struct Pool {
             (void)hipFree(p);
     }
     hipError_t alloc(void** p, size_t bytes) {
-        return host_ ? hipHostMalloc(p, bytes, hipHostMallocDefault) : hipMalloc(p, bytes);
+        return host_ ? hipHostMalloc(p, bytes, flags_) : hipMalloc(p, bytes);
     }
     void* get(size_t bytes) {
         bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
@@ -120,7 +122,10 @@ struct bpx_ctx {
     uint32_t next_lane = 0;                     // consecutive scans may overlap on the device
     uint32_t seg_hint = 0;             // per-wave record segment size a previous scan needed
     Pool pool;
-    Pool pinned{true};
+    Pool pinned{hipHostMallocDefault};
+    // counters the scan kernel writes to directly: coherent, so its stores
+    // go to host memory as they are made, not to the device caches
+    Pool coherent{hipHostMallocMapped | hipHostMallocCoherent};
 };
 
 struct bpx_db {
@@ -195,7 +200,7 @@ struct bpx_result {
     int reruns = 0;
     // device
     uint32_t* d_cnt = nullptr;
-    int32_t* d_stats = nullptr;
+    int32_t* d_sstats = nullptr;           // device address of h_sstats
     uint32_t* d_run_loc = nullptr;
     uint32_t* d_run_len = nullptr;
     bpx::SegRecord* d_seg = nullptr;
@@ -209,7 +214,6 @@ struct bpx_result {
     uint32_t* d_order = nullptr;           // owned only for sub-range scans
     uint64_t* d_poff = nullptr;            // (sub-range) descriptors in processing order
     uint32_t* d_plen = nullptr;
-    int32_t* d_stats_k = nullptr;          // counters per (query, processing position)
     bpx::WorkItem* d_items = nullptr;
     // host (pinned)
     unsigned long long* h_meta = nullptr;
@@ -527,9 +531,6 @@ void result_release(bpx_result* r, bool keep_order) {
     auto& pool = r->ctx->pool;
     auto& pinned = r->ctx->pinned;
     pool.put(r->d_cnt);
-    pool.put(r->d_stats);
-    pool.put(r->d_stats_k);
-    r->d_stats_k = nullptr;
     pool.put(r->d_run_loc);
     pool.put(r->d_run_len);
     pool.put(r->d_seg);
@@ -552,10 +553,10 @@ void result_release(bpx_result* r, bool keep_order) {
     pinned.put(r->h_records);
     pinned.put(r->h_subjects);
     pinned.put(r->h_queries);
-    pinned.put(r->h_sstats);
+    r->ctx->coherent.put(r->h_sstats);
     pinned.put(r->h_meta);
     r->d_cnt = nullptr;
-    r->d_stats = nullptr;
+    r->d_sstats = nullptr;
     r->d_run_loc = nullptr;
     r->d_run_len = nullptr;
     r->d_seg = nullptr;
@@ -588,9 +589,13 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     if (r->seg_total > 0xFFFFFFFFull) return fail(BPX_E_BUDGET, "record segments exceed 2^32 records");
     const uint64_t ntiles = (pairs + bpx::kTile - 1) / bpx::kTile;
     r->d_cnt = (uint32_t*)pool.get(pairs * 4);
-    if (r->flags & BPX_OUT_SUBJECT_STATS) {
-        r->d_stats = (int32_t*)pool.get(pairs * 12);
-        r->d_stats_k = (int32_t*)pool.get(pairs * 12);
+    if ((r->flags & BPX_OUT_SUBJECT_STATS) && pairs) {
+        r->h_sstats = (bpx_subject_stats*)r->ctx->coherent.get(pairs * sizeof(bpx_subject_stats));
+        void* dev = nullptr;
+        if (!r->h_sstats || hipHostGetDevicePointer(&dev, r->h_sstats, 0) != hipSuccess || !dev)
+            return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed (%llu query-subject pairs)",
+                        (unsigned long long)pairs);
+        r->d_sstats = static_cast<int32_t*>(dev);
     }
     r->d_run_loc = (uint32_t*)pool.get(runs * 4);
     r->d_run_len = (uint32_t*)pool.get(runs * 4);
@@ -603,8 +608,7 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
     r->d_work = (uint32_t*)pool.get(kMaxClasses * 4);
     r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
-    if (!r->d_cnt || ((r->flags & BPX_OUT_SUBJECT_STATS) && (!r->d_stats || !r->d_stats_k)) || !r->d_run_loc ||
-        !r->d_run_len ||
+    if (!r->d_cnt || !r->d_run_loc || !r->d_run_len ||
         !r->d_seg || !r->d_tile || !r->d_records || !r->d_subjects || !r->d_queries || !r->d_meta || !r->d_items ||
         !r->d_work ||
         !r->h_meta)
@@ -614,8 +618,9 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
 }
 
 // The scan's device work on the context stream: counter resets (fill
-// kernels, no memset nodes), every launch class, compaction and the copies of
-// the meta words / subject counters to pinned host memory.  `events` brackets
+// kernels, no memset nodes), every launch class, compaction and the copy of
+// the meta words to pinned host memory (the scan kernel writes the subject
+// counters to host memory itself).  `events` brackets
 // the word-finder launches with the timing events (not when captured into a
 // graph: then the caller brackets the graph launch).  The work items must be
 // on the device already (upload_items).
@@ -657,7 +662,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
         sp.nitems = c.nitems;
         sp.work_counter = r->d_work + ci;
         sp.cnt = r->d_cnt;
-        sp.stats = r->d_stats_k;
+        sp.stats = r->d_sstats;
         sp.run_loc = r->d_run_loc;
         sp.run_len = r->d_run_len;
         sp.ustats = ustats;
@@ -707,13 +712,6 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
     cp.overflow = overflow;
     HIP_TRY(bpx::launch_compact(cp, st));
     HIP_TRY(hipMemcpyAsync(r->h_meta, r->d_meta, (kMetaWords + 5ull * set.nq) * 8, hipMemcpyDeviceToHost, st));
-    if ((r->flags & BPX_OUT_SUBJECT_STATS) && r->count) {
-        const uint64_t pairs = (uint64_t)set.nq * r->count;
-        if (!r->h_sstats) r->h_sstats = (bpx_subject_stats*)ctx->pinned.get(pairs * sizeof(bpx_subject_stats));
-        if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
-        HIP_TRY(bpx::launch_permute_stats(r->d_stats_k, r->d_stats, order, r->first, r->count, pairs, st));
-        HIP_TRY(hipMemcpyAsync(r->h_sstats, r->d_stats, pairs * sizeof(bpx_subject_stats), hipMemcpyDeviceToHost, st));
-    }
     return BPX_OK;
 }
 
@@ -741,17 +739,12 @@ void graph_release(bpx_result* r) {
 // Captures the device work of the scan into a HIP graph and launches it,
 // bracketed by the timing events.
 int graph_capture_and_launch(bpx_result* r) {
-    bpx_ctx* ctx = r->ctx;
     const uint32_t* order = r->d_order ? r->d_order : r->db->d_order;
     graph_release(r);
     int rc = upload_items(r);
     if (rc) return rc;
     // everything that allocates or sets kernel attributes happens before the
-    // capture: the pinned counter buffer and each launch's LDS size
-    if ((r->flags & BPX_OUT_SUBJECT_STATS) && r->count && !r->h_sstats) {
-        r->h_sstats = (bpx_subject_stats*)ctx->pinned.get((uint64_t)r->set->nq * r->count * sizeof(bpx_subject_stats));
-        if (!r->h_sstats) return fail(BPX_E_NOMEM, "pinned allocation for subject stats failed");
-    }
+    // capture (the buffers, in alloc_buffers, and each launch's LDS size)
     for (const auto& c : r->classes) {
         bpx::ScanParams sp{};
         sp.word = r->set->word;
