@@ -173,11 +173,11 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     value = db.num_residues / (ms_per_step / 1000.0)
 
-    # roofline of the word-finder kernel: algorithmic bytes per launch =
-    # residues (1 B) + subject descriptors (12 B) read + per-subject counters
-    # (12 B) + records (24 B) written
+    # roofline of the word-finder kernel: algorithmic HBM bytes per launch =
+    # residues (1 B) + subject descriptors (12 B) read + records (24 B)
+    # written; the per-subject counters (12 B) go to pinned host memory
     kern = float(np.mean(kernel_ms))
-    bytes_alg = db.num_residues + 12 * db.num_subjects + 12 * db.num_subjects + 24 * nrec
+    bytes_alg = db.num_residues + 12 * db.num_subjects + 24 * nrec
     achieved = bytes_alg / (kern / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
