@@ -289,6 +289,9 @@ def batched(args):
 
     kern = float(np.mean(kernel_ms))
     passes = info["groups"]
+    # every group pass reads the residues and descriptors; each (query,
+    # subject) pair's record count is zero-filled, summed and turned into its
+    # CSR offset in place (12 B); records are 24 B
     bytes_alg = passes * (db.num_residues + 12 * db.num_subjects) + 12 * db.num_subjects * len(qs) + 24 * nrec
     achieved = bytes_alg / (kern / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
