@@ -170,6 +170,43 @@ def test_edge_lengths_and_subrange(ctx):
     assert len(st) == 0 and len(recs) == 0 and ustats["lookup_hits"] == 0
 
 
+def test_overlapping_scans_keep_their_own_counters(ctx):
+    """The kernel writes per-subject counters straight to pinned host memory.
+    Two scans in flight at once (the context's two stream lanes) each fill
+    their own buffer; a third scan, started while the second is pending,
+    reuses the buffer the first released."""
+    rng = np.random.default_rng(21)
+    qa, qb = rng.choice(STD20, 300), rng.choice(STD20, 180)
+    subjects = [np.concatenate([rng.choice(STD20, int(rng.integers(0, 400))),
+                                (qa if i % 2 else qb)[: int(rng.integers(20, 180))]]) for i in range(3000)]
+    res, offs, lens = pack(subjects)
+    say("overlapping scans: 3,000 subjects, queries of 300 and 180")
+    db = ctx.load_db(res, offs[:-1], lens)
+    da, dq_b = ctx.prepare_query(qa), ctx.prepare_query(qb)
+
+    def grab(r):
+        r.wait()
+        recs, subs = r.records()
+        st = r.subject_stats()
+        _, u = r.summary()
+        r.close()
+        return st, recs, subs, u, None
+
+    ra = bpx.scan(ctx, da, db, subject_stats=True)
+    rb = bpx.scan(ctx, dq_b, db, subject_stats=True)
+    ga = grab(ra)
+    rc = bpx.scan(ctx, da, db, subject_stats=True)
+    gb = grab(rb)
+    gc = grab(rc)
+    orc_a = oracle_run(qa, res, offs, lens)
+    assert_parity(ga, orc_a)
+    assert_parity(gb, oracle_run(qb, res, offs, lens))
+    assert_parity(gc, orc_a)
+    da.close()
+    dq_b.close()
+    db.close()
+
+
 def test_long_subjects(ctx):
     """Subjects longer than 65,536 residues exercise the 16-bit ring rebase."""
     rng = np.random.default_rng(6)
