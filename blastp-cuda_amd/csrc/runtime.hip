@@ -11,6 +11,7 @@
 #include <tuple>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -880,6 +881,115 @@ hipError_t upload_order(bpx_db* db) {
     if (e == hipSuccess) e = hipMemcpy(db->d_plen, plen.data(), n * 4, hipMemcpyHostToDevice);
     return e;
 }
+
+// Subject ids longest first, ties by id: the dynamic work queue then behaves
+// like longest-processing-time-first list scheduling.  A counting sort by
+// length (stable), or a stable comparison sort for very long subjects.
+std::vector<uint32_t> longest_first(const std::vector<uint32_t>& len, uint32_t maxlen) {
+    const size_t n = len.size();
+    std::vector<uint32_t> order(n);
+    if (maxlen <= (1u << 22)) {
+        std::vector<uint32_t> start((size_t)maxlen + 2, 0);
+        for (uint32_t l : len) ++start[(size_t)(maxlen - l) + 1];
+        for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+        for (size_t i = 0; i < n; ++i) order[start[maxlen - len[i]]++] = (uint32_t)i;
+    } else {
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return len[a] > len[b]; });
+    }
+    return order;
+}
+
+// Repacks the residues into the arena layout and uploads it: chunks of whole
+// subjects are filled by host threads into one of two pinned staging buffers
+// while the other one's copy runs.  Residue codes are checked on the way; the
+// first bad one (lowest subject, then position) is reported.
+int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const uint64_t* offsets,
+                 const uint32_t* lengths, const std::vector<uint64_t>& off, uint64_t arena) {
+    const uint32_t n = (uint32_t)off.size();
+    const uint64_t kChunk = 64ull << 20;
+    uint64_t biggest = kArenaTail;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t end = i + 1 < n ? off[i + 1] : arena;
+        biggest = std::max(biggest, end - off[i]);
+    }
+    const uint64_t cap = std::min(arena, std::max(kChunk, biggest));
+    // staging buffers of its own (not the context's pools, which a server
+    // thread on the same context may be using)
+    uint8_t* stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int rc = BPX_OK;
+    for (auto& p : stage)
+        if (rc == BPX_OK && hipHostMalloc((void**)&p, cap, hipHostMallocDefault) != hipSuccess)
+            rc = fail(BPX_E_NOMEM, "pinned staging allocation of %llu bytes failed", (unsigned long long)cap);
+    for (auto& ev : done)
+        if (rc == BPX_OK && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            rc = fail(BPX_E_HIP, "event creation failed");
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    uint32_t i0 = 0;
+    for (int c = 0; rc == BPX_OK && (i0 < n || c == 0); ++c) {
+        // whole subjects [i0, i1) of at most cap bytes; the last chunk ends at the arena's end
+        uint32_t i1 = i0;
+        const uint64_t base = i0 < n ? off[i0] : 0;
+        while (i1 < n && (i1 + 1 < n ? off[i1 + 1] : arena) - base <= cap) ++i1;
+        if (i1 == i0 && i0 < n) i1 = i0 + 1;
+        const uint64_t top = i1 < n ? off[i1] : arena;
+        uint8_t* buf = stage[c & 1];
+        if (c >= 2 && hipEventSynchronize(done[c & 1]) != hipSuccess) {
+            rc = fail(BPX_E_HIP, "arena upload failed");
+            break;
+        }
+        std::vector<uint64_t> bad(nthreads, ~0ull);  // (subject << 32 | position) of each thread's first bad code
+        auto work = [&](unsigned t) {
+            const uint32_t a = i0 + (uint32_t)((uint64_t)(i1 - i0) * t / nthreads);
+            const uint32_t b = i0 + (uint32_t)((uint64_t)(i1 - i0) * (t + 1) / nthreads);
+            for (uint32_t i = a; i < b; ++i) {
+                const uint8_t* src = residues + offsets[i];
+                const uint32_t L = lengths[i];
+                unsigned flag = 0;
+                for (uint32_t k = 0; k < L; ++k) flag |= src[k] >= BPX_ALPHABET;
+                if (flag && bad[t] == ~0ull) {
+                    uint32_t k = 0;
+                    while (src[k] < BPX_ALPHABET) ++k;
+                    bad[t] = (uint64_t)i << 32 | k;
+                }
+                uint8_t* dst = buf + (off[i] - base);
+                std::memcpy(dst, src, L);
+                const uint64_t end = (i + 1 < n ? off[i + 1] : arena) - off[i];
+                std::memset(dst + L, 0, end - L);
+            }
+        };
+        if (i0 == n) {
+            std::memset(buf, 0, arena - base);  // no subjects: the zero tail alone
+        } else if (i1 - i0 < 64 * nthreads) {
+            for (unsigned t = 0; t < nthreads; ++t) work(t);
+        } else {
+            std::vector<std::thread> pool;
+            for (unsigned t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+            work(0);
+            for (auto& th : pool) th.join();
+        }
+        const uint64_t first_bad = *std::min_element(bad.begin(), bad.end());
+        if (first_bad != ~0ull) {
+            const uint32_t i = (uint32_t)(first_bad >> 32), k = (uint32_t)first_bad;
+            rc = fail(BPX_E_ARG, "subject %u position %u: residue code %u >= 28", i, k, residues[offsets[i] + k]);
+            break;
+        }
+        if (hipMemcpyAsync(d_arena + base, buf, top - base, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipEventRecord(done[c & 1], ctx->stream) != hipSuccess) {
+            rc = fail(BPX_E_HIP, "arena upload failed");
+            break;
+        }
+        i0 = i1;
+        if (i0 == n && top == arena) break;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == BPX_OK) rc = fail(BPX_E_HIP, "arena upload failed");
+    for (auto& ev : done)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto* p : stage)
+        if (p) (void)hipHostFree(p);
+    return rc;
+}
 }  // namespace
 
 extern "C" {
@@ -949,25 +1059,19 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
     *out = nullptr;
     if (num_subjects && (!offsets || !lengths || !residues)) return fail(BPX_E_ARG, "NULL arrays");
     HIP_TRY(hipSetDevice(ctx->device));
-    // Repack into the device layout: 16-byte-aligned subject starts, zero
-    // padding, and a readable zero tail.
+    // The device layout: 128-byte-aligned subject starts, zero padding, and
+    // a readable zero tail.
     std::vector<uint64_t> off(num_subjects);
     uint64_t at = 0, nres = 0;
+    uint32_t maxlen = 0;
     for (uint32_t i = 0; i < num_subjects; ++i) {
         if (lengths[i] >= (1u << 28)) return fail(BPX_E_ARG, "subject %u longer than 2^28 residues", i);
         off[i] = at;
         at += ((uint64_t)lengths[i] + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
         nres += lengths[i];
+        maxlen = std::max(maxlen, lengths[i]);
     }
     const uint64_t arena = at + kArenaTail;
-    std::vector<uint8_t> host(arena, 0);
-    for (uint32_t i = 0; i < num_subjects; ++i) {
-        const uint8_t* src = residues + offsets[i];
-        for (uint32_t k = 0; k < lengths[i]; ++k)
-            if (src[k] >= BPX_ALPHABET)
-                return fail(BPX_E_ARG, "subject %u position %u: residue code %u >= 28", i, k, src[k]);
-        std::memcpy(host.data() + off[i], src, lengths[i]);
-    }
     auto* db = new (std::nothrow) bpx_db;
     if (!db) return fail(BPX_E_NOMEM, "host allocation failed");
     db->ctx = ctx;
@@ -975,12 +1079,7 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
     db->nres = nres;
     db->arena_bytes = arena;
     db->h_len.assign(lengths, lengths + num_subjects);
-    db->h_order.resize(num_subjects);
-    std::iota(db->h_order.begin(), db->h_order.end(), 0u);
-    // longest first (ties by index): the dynamic work queue then behaves like
-    // longest-processing-time-first list scheduling
-    std::stable_sort(db->h_order.begin(), db->h_order.end(),
-                     [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
+    db->h_order = longest_first(db->h_len, maxlen);
     const size_t n1 = std::max<uint32_t>(num_subjects, 1);
     if (hipMalloc(&db->d_arena, arena) != hipSuccess || hipMalloc(&db->d_off, n1 * 8) != hipSuccess ||
         hipMalloc(&db->d_len, n1 * 4) != hipSuccess || hipMalloc(&db->d_order, n1 * 4) != hipSuccess ||
@@ -988,8 +1087,13 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
         bpx_db_destroy(db);
         return fail(BPX_E_NOMEM, "device allocation of %llu-byte arena failed", (unsigned long long)arena);
     }
-    hipError_t e = hipMemcpy(db->d_arena, host.data(), arena, hipMemcpyHostToDevice);
-    if (e == hipSuccess && num_subjects) {
+    int rc = upload_arena(ctx, db->d_arena, residues, offsets, lengths, off, arena);
+    if (rc) {
+        bpx_db_destroy(db);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (num_subjects) {
         e = hipMemcpy(db->d_off, off.data(), (size_t)num_subjects * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(db->d_len, lengths, (size_t)num_subjects * 4, hipMemcpyHostToDevice);
     }

@@ -322,6 +322,28 @@ def test_infeasible_launch_is_refused(ctx):
     assert rc == -5 and lds > ctx.info()["lds_bytes"]
 
 
+def test_database_load_checks_codes_and_handles_empty(ctx):
+    """The arena is repacked by host threads in chunks; the first bad residue
+    code (lowest subject, then position) is reported, and an empty volume
+    loads and scans to nothing."""
+    rng = np.random.default_rng(22)
+    subjects = [rng.choice(STD20, int(rng.integers(1, 400))) for _ in range(20_000)]
+    subjects[9_000][3] = 28
+    subjects[15_000][0] = 200
+    subjects[9_000][7] = 31
+    res, offs, lens = pack(subjects)
+    with pytest.raises(bpx.BpxError, match="subject 9000 position 3: residue code 28"):
+        ctx.load_db(res, offs[:-1], lens)
+    empty = ctx.load_db(np.zeros(0, np.uint8), np.zeros(0, np.uint64), np.zeros(0, np.uint32))
+    q = ctx.prepare_query(rng.choice(STD20, 100))
+    r = bpx.scan(ctx, q, empty, subject_stats=True).wait()
+    n, ust = r.summary()
+    assert n == 0 and len(r.subject_stats()) == 0 and ust["lookup_hits"] == 0
+    r.close()
+    q.close()
+    empty.close()
+
+
 def workload_db(which):
     if which == 1:
         q = bpx.synth_query(0, 0, 300)
