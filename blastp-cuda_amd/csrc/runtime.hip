@@ -964,9 +964,17 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
         } else if (i1 - i0 < 64 * nthreads) {
             for (unsigned t = 0; t < nthreads; ++t) work(t);
         } else {
+            // partitions a thread could not be started for run here
             std::vector<std::thread> pool;
-            for (unsigned t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
-            work(0);
+            std::vector<unsigned> here{0};
+            for (unsigned t = 1; t < nthreads; ++t) {
+                try {
+                    pool.emplace_back(work, t);
+                } catch (...) {
+                    here.push_back(t);
+                }
+            }
+            for (unsigned t : here) work(t);
             for (auto& th : pool) th.join();
         }
         const uint64_t first_bad = *std::min_element(bad.begin(), bad.end());
