@@ -900,20 +900,30 @@ std::vector<uint32_t> longest_first(const std::vector<uint32_t>& len, uint32_t m
     return order;
 }
 
-// Repacks the residues into the arena layout and uploads it: chunks of whole
-// subjects are filled by host threads into one of two pinned staging buffers
-// while the other one's copy runs.  Residue codes are checked on the way; the
-// first bad one (lowest subject, then position) is reported.
+// A host array to copy to the device through the staging buffers.
+struct HostCopy {
+    void* dst;
+    const void* src;
+    uint64_t bytes;
+};
+
+// Repacks the residues into the arena layout and uploads it, then the
+// descriptor arrays: chunks of whole subjects are filled by host threads into
+// one of two pinned staging buffers while the other one's copy runs.
+// Residue codes are checked on the way; the first bad one (lowest subject,
+// then position) is reported.
 int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const uint64_t* offsets,
-                 const uint32_t* lengths, const std::vector<uint64_t>& off, uint64_t arena) {
+                 const uint32_t* lengths, const std::vector<uint64_t>& off, uint64_t arena,
+                 const std::vector<HostCopy>& extra) {
     const uint32_t n = (uint32_t)off.size();
     const uint64_t kChunk = 64ull << 20;
-    uint64_t biggest = kArenaTail;
+    uint64_t biggest = kArenaTail, need = arena;
     for (uint32_t i = 0; i < n; ++i) {
         const uint64_t end = i + 1 < n ? off[i + 1] : arena;
         biggest = std::max(biggest, end - off[i]);
     }
-    const uint64_t cap = std::min(arena, std::max(kChunk, biggest));
+    for (const auto& h : extra) need = std::max(need, h.bytes);
+    const uint64_t cap = std::max(biggest, std::min(kChunk, need));
     // staging buffers of its own (not the context's pools, which a server
     // thread on the same context may be using)
     uint8_t* stage[2] = {nullptr, nullptr};
@@ -925,17 +935,29 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
     for (auto& ev : done)
         if (rc == BPX_OK && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
             rc = fail(BPX_E_HIP, "event creation failed");
+    uint64_t c = 0;  // copies submitted; copy c uses stage[c & 1]
+    auto next_buf = [&]() -> uint8_t* {
+        if (c >= 2 && hipEventSynchronize(done[c & 1]) != hipSuccess) return nullptr;
+        return stage[c & 1];
+    };
+    auto submit = [&](void* dst, uint64_t bytes) -> bool {
+        if (hipMemcpyAsync(dst, stage[c & 1], bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipEventRecord(done[c & 1], ctx->stream) != hipSuccess)
+            return false;
+        ++c;
+        return true;
+    };
     const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     uint32_t i0 = 0;
-    for (int c = 0; rc == BPX_OK && (i0 < n || c == 0); ++c) {
+    while (rc == BPX_OK) {
         // whole subjects [i0, i1) of at most cap bytes; the last chunk ends at the arena's end
         uint32_t i1 = i0;
         const uint64_t base = i0 < n ? off[i0] : 0;
         while (i1 < n && (i1 + 1 < n ? off[i1 + 1] : arena) - base <= cap) ++i1;
         if (i1 == i0 && i0 < n) i1 = i0 + 1;
         const uint64_t top = i1 < n ? off[i1] : arena;
-        uint8_t* buf = stage[c & 1];
-        if (c >= 2 && hipEventSynchronize(done[c & 1]) != hipSuccess) {
+        uint8_t* buf = next_buf();
+        if (!buf) {
             rc = fail(BPX_E_HIP, "arena upload failed");
             break;
         }
@@ -944,17 +966,20 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
             const uint32_t a = i0 + (uint32_t)((uint64_t)(i1 - i0) * t / nthreads);
             const uint32_t b = i0 + (uint32_t)((uint64_t)(i1 - i0) * (t + 1) / nthreads);
             for (uint32_t i = a; i < b; ++i) {
-                const uint8_t* src = residues + offsets[i];
+                const uint8_t* __restrict__ src = residues + offsets[i];
+                uint8_t* __restrict__ dst = buf + (off[i] - base);
                 const uint32_t L = lengths[i];
                 unsigned flag = 0;
-                for (uint32_t k = 0; k < L; ++k) flag |= src[k] >= BPX_ALPHABET;
+                for (uint32_t k = 0; k < L; ++k) {  // copy and check in one pass
+                    const uint8_t v = src[k];
+                    dst[k] = v;
+                    flag |= v >= BPX_ALPHABET;
+                }
                 if (flag && bad[t] == ~0ull) {
                     uint32_t k = 0;
                     while (src[k] < BPX_ALPHABET) ++k;
                     bad[t] = (uint64_t)i << 32 | k;
                 }
-                uint8_t* dst = buf + (off[i] - base);
-                std::memcpy(dst, src, L);
                 const uint64_t end = (i + 1 < n ? off[i + 1] : arena) - off[i];
                 std::memset(dst + L, 0, end - L);
             }
@@ -983,14 +1008,20 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
             rc = fail(BPX_E_ARG, "subject %u position %u: residue code %u >= 28", i, k, residues[offsets[i] + k]);
             break;
         }
-        if (hipMemcpyAsync(d_arena + base, buf, top - base, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-            hipEventRecord(done[c & 1], ctx->stream) != hipSuccess) {
+        if (!submit(d_arena + base, top - base)) {
             rc = fail(BPX_E_HIP, "arena upload failed");
             break;
         }
         i0 = i1;
-        if (i0 == n && top == arena) break;
+        if (i0 >= n) break;
     }
+    for (const auto& h : extra)
+        for (uint64_t at = 0; rc == BPX_OK && at < h.bytes; at += cap) {
+            const uint64_t b = std::min(cap, h.bytes - at);
+            uint8_t* buf = next_buf();
+            if (buf) std::memcpy(buf, static_cast<const uint8_t*>(h.src) + at, b);
+            if (!buf || !submit(static_cast<uint8_t*>(h.dst) + at, b)) rc = fail(BPX_E_HIP, "descriptor upload failed");
+        }
     if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == BPX_OK) rc = fail(BPX_E_HIP, "arena upload failed");
     for (auto& ev : done)
         if (ev) (void)hipEventDestroy(ev);
@@ -1095,22 +1126,25 @@ int bpx_db_load(bpx_ctx* ctx, const uint8_t* residues, const uint64_t* offsets, 
         bpx_db_destroy(db);
         return fail(BPX_E_NOMEM, "device allocation of %llu-byte arena failed", (unsigned long long)arena);
     }
-    int rc = upload_arena(ctx, db->d_arena, residues, offsets, lengths, off, arena);
+    // descriptors by subject id and, in processing order, the order itself
+    // with each subject's offset and length
+    std::vector<uint64_t> poff(num_subjects);
+    std::vector<uint32_t> plen(num_subjects);
+    for (uint32_t k = 0; k < num_subjects; ++k) {
+        poff[k] = off[db->h_order[k]];
+        plen[k] = lengths[db->h_order[k]];
+    }
+    const std::vector<HostCopy> extra = {{db->d_off, off.data(), (uint64_t)num_subjects * 8},
+                                         {db->d_len, lengths, (uint64_t)num_subjects * 4},
+                                         {db->d_order, db->h_order.data(), (uint64_t)num_subjects * 4},
+                                         {db->d_poff, poff.data(), (uint64_t)num_subjects * 8},
+                                         {db->d_plen, plen.data(), (uint64_t)num_subjects * 4}};
+    int rc = upload_arena(ctx, db->d_arena, residues, offsets, lengths, off, arena, extra);
     if (rc) {
         bpx_db_destroy(db);
         return rc;
     }
-    hipError_t e = hipSuccess;
-    if (num_subjects) {
-        e = hipMemcpy(db->d_off, off.data(), (size_t)num_subjects * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(db->d_len, lengths, (size_t)num_subjects * 4, hipMemcpyHostToDevice);
-    }
     db->h_off = std::move(off);
-    if (e == hipSuccess && num_subjects) e = upload_order(db);
-    if (e != hipSuccess) {
-        bpx_db_destroy(db);
-        return fail(BPX_E_HIP, "arena upload failed: %s", hipGetErrorString(e));
-    }
     *out = db;
     return BPX_OK;
 }
