@@ -121,7 +121,7 @@ typedef struct bpx_batch_policy {
 } bpx_batch_policy;
 
 /* Output selection for a scan. */
-#define BPX_OUT_SUBJECT_STATS 1u /* copy per-subject counters to the host */
+#define BPX_OUT_SUBJECT_STATS 1u /* per-subject counters to the host (the kernel writes them to pinned memory) */
 #define BPX_OUT_GRAPH 2u         /* capture the scan's device work in a HIP graph (bpx_result_replay) */
 
 /* ---- device / context ------------------------------------------------- */
@@ -136,7 +136,9 @@ int bpx_context_info(const bpx_ctx *ctx, int32_t *num_cus, int64_t *lds_bytes,
 /* ---- database: a resident int8 residue arena ---------------------------
  * residues: ncbistdaa codes; subject i is residues[offsets[i] ..
  * offsets[i]+lengths[i]).  The library repacks it into its own aligned
- * arena on the device; the host arrays may be freed on return. */
+ * arena on the device (with up to 16 host threads, through pinned staging
+ * buffers of its own) and checks every code; the host arrays may be freed
+ * on return. */
 int bpx_db_load(bpx_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
                 const uint32_t *lengths, uint32_t num_subjects, bpx_db **out);
 int bpx_db_info(const bpx_db *db, uint32_t *num_subjects, uint64_t *num_residues,
