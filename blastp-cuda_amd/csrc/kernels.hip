@@ -1039,9 +1039,11 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
     }
 }
 
-__global__ void k_fill_u32(uint32_t* dst, uint32_t value, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = value;
+// zero-fills up to kZeroRanges arrays in one launch
+__global__ void k_zero_u32(ZeroRanges z) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (int r = 0; r < z.count; ++r)
+        for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < z.n[r]; i += stride) z.dst[r][i] = 0u;
 }
 
 }  // namespace
@@ -1129,11 +1131,12 @@ hipError_t launch_compact(const CompactParams& p, hipStream_t stream) {
 }
 
 
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    const uint64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream, dst, value,
-                       n);
+hipError_t launch_zero_u32(const ZeroRanges& z, hipStream_t stream) {
+    uint64_t most = 0;
+    for (int r = 0; r < z.count; ++r) most = most > z.n[r] ? most : z.n[r];
+    if (most == 0) return hipSuccess;
+    const uint64_t blocks = (most + 255) / 256;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream, z);
     return hipGetLastError();
 }
 

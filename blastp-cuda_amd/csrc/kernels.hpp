@@ -149,6 +149,13 @@ struct CompactParams {
 constexpr uint32_t kTile = 1024;
 hipError_t launch_compact(const CompactParams& p, hipStream_t stream);
 
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t n, hipStream_t stream);
+constexpr int kZeroRanges = 4;
+struct ZeroRanges {
+    uint32_t* dst[kZeroRanges];
+    uint64_t n[kZeroRanges];
+    int count;
+};
+// zero-fills the ranges with one kernel launch (no memset nodes in a graph)
+hipError_t launch_zero_u32(const ZeroRanges& z, hipStream_t stream);
 
 }  // namespace bpx

@@ -630,11 +630,19 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
     const QuerySet& set = *r->set;
     const bpx_db* db = r->db;
     hipStream_t st = r->stream;
-    HIP_TRY(bpx::launch_fill_u32(reinterpret_cast<uint32_t*>(r->d_meta), 0u, 2 * (kMetaWords + 5ull * set.nq), st));
-    HIP_TRY(bpx::launch_fill_u32(r->d_work, 0u, kMaxClasses, st));
-    // the kernel writes counts and run lengths only for subjects with records
-    HIP_TRY(bpx::launch_fill_u32(r->d_cnt, 0u, (uint64_t)set.nq * r->count, st));
-    HIP_TRY(bpx::launch_fill_u32(r->d_run_len, 0u, (uint64_t)set.groups.size() * r->count, st));
+    // meta words, work counters, and the counts and run lengths, which the
+    // kernel writes only for subjects with records: one launch
+    bpx::ZeroRanges z{};
+    z.dst[0] = reinterpret_cast<uint32_t*>(r->d_meta);
+    z.n[0] = 2 * (kMetaWords + 5ull * set.nq);
+    z.dst[1] = r->d_work;
+    z.n[1] = kMaxClasses;
+    z.dst[2] = r->d_cnt;
+    z.n[2] = (uint64_t)set.nq * r->count;
+    z.dst[3] = r->d_run_len;
+    z.n[3] = (uint64_t)set.groups.size() * r->count;
+    z.count = 4;
+    HIP_TRY(bpx::launch_zero_u32(z, st));
     unsigned long long* ustats = r->d_meta + kMetaWords;
     uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
     if (events && start_event) HIP_TRY(hipEventRecord(r->ev[0], st));
@@ -1385,15 +1393,18 @@ int bpx_result_wait(bpx_result* r) {
     }
     r->total = r->h_meta[0];
     if (r->total) {
+        // one query: every query index is 0, nothing to copy
+        const bool multi = r->set->nq > 1;
         r->h_records = (bpx_init_hsp*)ctx->pinned.get(r->total * sizeof(bpx_init_hsp));
         r->h_subjects = (uint32_t*)ctx->pinned.get(r->total * 4);
-        r->h_queries = (uint32_t*)ctx->pinned.get(r->total * 4);
-        if (!r->h_records || !r->h_subjects || !r->h_queries)
+        r->h_queries = multi ? (uint32_t*)ctx->pinned.get(r->total * 4) : nullptr;
+        if (!r->h_records || !r->h_subjects || (multi && !r->h_queries))
             return fail(BPX_E_NOMEM, "pinned allocation for records failed");
         HIP_TRY(hipMemcpyAsync(r->h_records, r->d_records, r->total * sizeof(bpx_init_hsp), hipMemcpyDeviceToHost,
                                r->stream));
         HIP_TRY(hipMemcpyAsync(r->h_subjects, r->d_subjects, r->total * 4, hipMemcpyDeviceToHost, r->stream));
-        HIP_TRY(hipMemcpyAsync(r->h_queries, r->d_queries, r->total * 4, hipMemcpyDeviceToHost, r->stream));
+        if (multi)
+            HIP_TRY(hipMemcpyAsync(r->h_queries, r->d_queries, r->total * 4, hipMemcpyDeviceToHost, r->stream));
     }
     HIP_TRY(hipEventRecord(r->ev[2], r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
@@ -1463,7 +1474,12 @@ int bpx_result_records(const bpx_result* r, bpx_init_hsp* records, uint32_t* sub
                     (unsigned long long)r->total);
     if (records && r->total) std::memcpy(records, r->h_records, r->total * sizeof(bpx_init_hsp));
     if (subjects && r->total) std::memcpy(subjects, r->h_subjects, r->total * 4);
-    if (queries && r->total) std::memcpy(queries, r->h_queries, r->total * 4);
+    if (queries && r->total) {
+        if (r->h_queries)
+            std::memcpy(queries, r->h_queries, r->total * 4);
+        else
+            std::memset(queries, 0, r->total * 4);
+    }
     return BPX_OK;
 }
 
