@@ -411,12 +411,16 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t cur_group = 0xFFFFFFFFu;
     GroupInfo g{};
 
+    uint32_t pass = 0;               // flat: items in order, then done
+    uint32_t c_next = 0, c_end = 0;  // flat: this wave's claimed subjects (wave-uniform)
     for (;;) {
         __syncthreads();
         if (tid == 0) {
-            ctl[0] = atomicAdd(p.work_counter, 1u);
+            ctl[0] = p.flat ? min(pass, p.nitems) : atomicAdd(p.work_counter, 1u);
             ctl[1] = 0;
         }
+        ++pass;
+        c_next = c_end = 0;
         __syncthreads();
         const uint32_t item = ctl[0];
         if (item >= p.nitems) {
@@ -463,8 +467,20 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
 
         for (;;) {
             uint32_t k = 0;
-            if (lane == 0) k = atomicAdd(&ctl[1], 1u);
-            k = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
+            if (p.flat) {
+                // a chunk of the item's subjects from its device-wide counter:
+                // every subject goes to exactly one wave, in processing order
+                if (c_next == c_end) {
+                    uint32_t c = 0;
+                    if (lane == 0) c = atomicAdd(&p.flat_counters[item], p.flat);
+                    c_next = (uint32_t)__builtin_amdgcn_readlane((int)c, 0);
+                    c_end = c_next + p.flat;
+                }
+                k = c_next++;
+            } else {
+                if (lane == 0) k = atomicAdd(&ctl[1], 1u);
+                k = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
+            }
             if (k >= w.count) break;
             // descriptors are stored in processing order: consecutive
             // subjects of an item read consecutive entries

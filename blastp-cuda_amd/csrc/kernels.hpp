@@ -76,6 +76,12 @@ struct ScanParams {
     const WorkItem* items;
     uint32_t nitems;
     uint32_t* work_counter;
+    // nonzero: every workgroup works through all items in order (one per
+    // group), each wavefront claiming `flat` subjects at a time from the
+    // item's counter in flat_counters; zero: workgroups claim items from
+    // *work_counter and their waves claim the item's subjects one at a time
+    uint32_t flat;
+    uint32_t* flat_counters;
     // outputs
     uint32_t* cnt;              // saved records per (qid, slot): qid * count + slot; zero-filled
     int32_t* stats;             // optional: 3 per (qid, slot): qid * count + slot, in pinned host memory

@@ -52,6 +52,7 @@ constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last sub
 constexpr int kMaxClasses = 128;       // launches per scan: (table placement, multi, waves) bins x slot width (<= 2 x 2 x 13 x 2)
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 constexpr uint32_t kSegFloor = 1024;   // smallest per-wave record segment (64 B per record slot)
+constexpr uint32_t kFlatChunk = 16;    // subjects a wavefront claims at a time
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
 // do not call hipMalloc/hipFree.  Host pools pass their hipHostMalloc flags.
@@ -225,7 +226,7 @@ struct bpx_result {
     uint64_t total = 0;
     // timing
     hipStream_t stream = nullptr;    // the context lane this scan runs on
-    uint32_t* d_work = nullptr;      // work counters, one per launch class
+    uint32_t* d_work = nullptr;      // item counters, one per launch class, then one subject counter per item
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     // graph mode (BPX_OUT_GRAPH): the captured device work of the scan
     hipGraph_t graph = nullptr;
@@ -235,6 +236,7 @@ struct bpx_result {
     // launch plan
     struct Class {
         bool tab_lds = false, multi = false, narrow = true;
+        uint32_t flat = 0;  // subjects a wave claims at a time (0: work items)
         int waves = 0;
         bpx::LdsPlan plan{};
         uint32_t item_first = 0, nitems = 0;
@@ -516,8 +518,15 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             r->lds_need = std::max(r->lds_need, c.plan.total);
             c.diag_cap = std::max<uint32_t>(a.diag, 1);
             c.item_first = (uint32_t)r->items.size();
-            for (uint32_t gi : a.groups)
-                for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
+            if (L.subjects_per_item <= 0) {
+                // one item per group; every workgroup works through them in
+                // order and its waves claim 16 subjects at a time device-wide
+                for (uint32_t gi : a.groups) r->items.push_back({gi, lo, hi - lo, 0});
+                c.flat = kFlatChunk;
+            } else {
+                for (uint32_t gi : a.groups)
+                    for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
+            }
             c.nitems = (uint32_t)r->items.size() - c.item_first;
             r->classes.push_back(c);
         }
@@ -607,7 +616,7 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     r->d_queries = (uint32_t*)pool.get(r->seg_total * 4);
     r->d_meta = (unsigned long long*)pool.get((kMetaWords + 5ull * set.nq) * 8);
     r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
-    r->d_work = (uint32_t*)pool.get(kMaxClasses * 4);
+    r->d_work = (uint32_t*)pool.get((kMaxClasses + r->items.size()) * 4);
     r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
     if (!r->d_cnt || !r->d_run_loc || !r->d_run_len ||
         !r->d_seg || !r->d_tile || !r->d_records || !r->d_subjects || !r->d_queries || !r->d_meta || !r->d_items ||
@@ -636,7 +645,7 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
     z.dst[0] = reinterpret_cast<uint32_t*>(r->d_meta);
     z.n[0] = 2 * (kMetaWords + 5ull * set.nq);
     z.dst[1] = r->d_work;
-    z.n[1] = kMaxClasses;
+    z.n[1] = kMaxClasses + r->items.size();
     z.dst[2] = r->d_cnt;
     z.n[2] = (uint64_t)set.nq * r->count;
     z.dst[3] = r->d_run_len;
@@ -669,6 +678,8 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
         sp.window = set.window;
         sp.items = r->d_items + c.item_first;
         sp.nitems = c.nitems;
+        sp.flat = c.flat;
+        sp.flat_counters = r->d_work + kMaxClasses + c.item_first;
         sp.work_counter = r->d_work + ci;
         sp.cnt = r->d_cnt;
         sp.stats = r->d_sstats;
