@@ -52,7 +52,7 @@ constexpr uint32_t kArenaTail = 256;   // readable zero bytes after the last sub
 constexpr int kMaxClasses = 128;       // launches per scan: (table placement, multi, waves) bins x slot width (<= 2 x 2 x 13 x 2)
 constexpr uint32_t kMetaWords = 8;     // u64 words before the per-query stats
 constexpr uint32_t kSegFloor = 1024;   // smallest per-wave record segment (64 B per record slot)
-constexpr uint32_t kFlatChunk = 16;    // subjects a wavefront claims at a time
+constexpr uint32_t kFlatChunk = 16;    // most subjects a wavefront claims at a time
 
 // Size-keyed cache of device (or pinned host) allocations, so repeated scans
 // do not call hipMalloc/hipFree.  Host pools pass their hipHostMalloc flags.
@@ -520,9 +520,17 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             c.item_first = (uint32_t)r->items.size();
             if (L.subjects_per_item <= 0) {
                 // one item per group; every workgroup works through them in
-                // order and its waves claim 16 subjects at a time device-wide
+                // order and its waves claim subjects device-wide, a few at a
+                // time: about 1/16 of a wave's share (at most kFlatChunk), so
+                // the longest subjects, which go first, cannot pile up on a
+                // few waves, and the counters see few enough atomics.  256
+                // per-query scans of 500,000 subjects took 0.80 s at 8 at a
+                // time, 0.84 s at 16 and 1.35 s at 32; on 2,000,000 subjects
+                // 8, 16 and 32 are equal and 1 is 2.3x slower (one atomic per
+                // subject).
+                const uint64_t wv = (uint64_t)ctx->num_cus * L.groups_per_cu * waves;
+                c.flat = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kFlatChunk, (hi - lo) / (wv * 16)));
                 for (uint32_t gi : a.groups) r->items.push_back({gi, lo, hi - lo, 0});
-                c.flat = kFlatChunk;
             } else {
                 for (uint32_t gi : a.groups)
                     for (uint32_t f = lo; f < hi; f += R) r->items.push_back({gi, f, std::min(R, hi - f), 0});
