@@ -464,22 +464,15 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
     }
     r->classes.clear();
     r->items.clear();
-    // Work items are runs of R subjects of the processing order; by default
-    // about 8 per CU for one group, 4 per CU and group for several (64 to
-    // 4,096 subjects).  On the 2,000,000-subject scan 4 per CU took 13.3 ms,
-    // 8 per CU 11.2, 12 per CU 11.0, 16 per CU 11.1 and 32 per CU 11.8; on a
-    // 500,000-subject one (per query) 8 per CU beat 12 by 14 % and 5 by 4 %:
-    // with few items the first ones (the longest subjects) outlast the rest,
-    // with many the workgroup barrier between items costs.  A batched scan has items of
-    // every group to balance with, so the barrier decides: on the 1,024-
-    // query batch 4 per CU and group beat 8 by 2 %.  Equal-residue items
-    // (64 subjects at least) took 12.3 ms on the 2,000,000-subject scan:
-    // items of few long subjects leave waves idle at the barrier.
-    const uint32_t per_cu = set.groups.size() > 1 ? 4u : 8u;
-    const uint32_t R =
-        L.subjects_per_item > 0
-            ? (uint32_t)L.subjects_per_item
-            : std::max<uint32_t>(64u, std::min<uint32_t>(4096u, count / (uint32_t)std::max(1, ctx->num_cus * (int)per_cu) + 1));
+    // By default each launch class has one work item per group, and its
+    // subjects are claimed device-wide by the wavefronts (below): no
+    // workgroup barrier between items, so no wave waits for the slowest wave
+    // of an item.  Runs of 8 items per CU (barriers between them) took 11.2
+    // ms on the 2,000,000-subject scan against 10.6 ms; per-query scans of a
+    // 500,000-subject database took 12 % longer, a 1,024-query batch 2 %.
+    // An explicit subjects_per_item keeps runs of that many subjects, each
+    // claimed by one workgroup.
+    const uint32_t R = (uint32_t)std::max(1, L.subjects_per_item);
     // LDS bins first, then by waves, most first
     std::vector<std::pair<std::tuple<int, int, int>, Acc*>> order;
     for (auto& kv : bins) order.push_back({kv.first, &kv.second});

@@ -105,7 +105,8 @@ typedef struct bpx_ungapped_stats {
 typedef struct bpx_launch {
     int32_t waves_per_group;    /* wavefronts per workgroup (1..16); 0 = the most that fit */
     int32_t groups_per_cu;      /* resident workgroups per CU (grid = CUs x this) */
-    int32_t subjects_per_item;  /* subjects per dynamic work item */
+    int32_t subjects_per_item;  /* subjects per work item a workgroup claims; 0 = subjects claimed
+                                   by wavefronts device-wide, no items */
     int32_t queue_capacity;     /* per-wave LDS hit ring entries (power of two) */
     int32_t subject_buffer;     /* per-wave LDS subject buffer bytes (power of two >= 1024) */
     int32_t reserved;
