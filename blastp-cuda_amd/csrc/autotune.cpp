@@ -9,11 +9,10 @@
 //      share one pass over a subject, hence the per-wave diagonal-table
 //      size and the lookup-table placement;
 //   2. subject buffer bytes (per-wave LDS window over the subject).
-// Subjects per work item are left to the library default, which sizes items
-// for the full scan: a sample has too few subjects to rank item sizes (on
-// the 1,024-query batch a 5 % sample preferred 256 subjects per item, the
-// full scan 512).  Results do not depend on any of these, so the tuner only
-// reads timings.
+// The work distribution is left to the library default (wavefronts claim
+// subjects device-wide, in chunks sized for the scan at hand), which a
+// sample could not rank anyway.  Results do not depend on any of these, so
+// the tuner only reads timings.
 #include <algorithm>
 #include <cstring>
 #include <vector>
