@@ -1,0 +1,71 @@
+"""GPU parity at the full size of the skewed workload (the 1,024 batch
+queries x 500,000 subjects of 20-35,000 residues, 20 % compositionally
+biased): with the learned cost order installed, the batch equals the CPU
+oracle for every 32nd query and the same batch under the static
+longest-first order for all 1,024 queries."""
+import time
+
+import numpy as np
+import pytest
+
+import bpx
+from bpx import costsched
+from bpx import workloads as W
+from fullsize import THREADS, assert_same_as_oracle, assert_same_snapshot, query_slices, say, snapshot
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert bpx.available(), "no HIP device visible"
+    c = bpx.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def skewed(ctx):
+    t = time.perf_counter()
+    qs, db = W.config4(THREADS)
+    assert db.sha256 == W.SHA256[4]
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    sample = costsched.sample_queries(qs, 16)
+    model, counts, losses = costsched.fit(ctx, W.CONFIG4_SPEC, sample, batches=100, subjects=2000, epochs=8)
+    say(f"skewed workload: cost model trained, losses {[round(float(x), 4) for x in losses]} "
+        f"({time.perf_counter() - t:.1f} s)")
+    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    X = bpx.db_features(ctx, ddb, counts)
+    cost = costsched.predicted_cost(model, X, db.lengths, len(qs), len(sample), b.info()["groups"])
+    model.close()
+    bpx.db_set_cost(ddb, cost)
+    g = bpx.scan_batch(ctx, b, ddb, launch=launch, subject_stats=True, graph=True).wait()
+    snap = snapshot(g, len(qs), db.num_subjects)
+    g.close()
+    say(f"skewed workload: learned-order batch, {len(snap['recs'])} records ({time.perf_counter() - t:.1f} s)")
+    yield qs, db, ddb, b, launch, snap
+    b.close()
+    ddb.close()
+
+
+@pytest.mark.timeout(900)
+def test_skewed_learned_order_equals_oracle_every_32nd_query(skewed):
+    qs, db, _, _, _, snap = skewed
+    sl = query_slices(snap["qids"], len(qs))
+    t = time.perf_counter()
+    for i in range(0, len(qs), 32):
+        a, b = sl[i]
+        assert_same_as_oracle(qs[i], db, snap["recs"][a:b], snap["subs"][a:b], snap["st"][i], snap["qstats"][i])
+        if i % 256 == 0:
+            say(f"  query {i} ({len(qs[i])} aa): equal to the oracle ({time.perf_counter() - t:.1f} s)")
+    say(f"32 queries equal to the oracle ({time.perf_counter() - t:.1f} s)")
+
+
+@pytest.mark.timeout(900)
+def test_skewed_learned_order_equals_longest_first(ctx, skewed):
+    qs, db, ddb, b, launch, snap = skewed
+    bpx.db_set_cost(ddb, None)
+    r = bpx.scan_batch(ctx, b, ddb, launch=launch, subject_stats=True).wait()
+    assert_same_snapshot(snapshot(r, len(qs), db.num_subjects), snap)
+    r.close()

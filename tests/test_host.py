@@ -132,3 +132,20 @@ def test_package_fails_loudly_without_the_library(tmp_path):
     shutil.copy(os.path.join(os.path.dirname(bpx.__file__), "__init__.py"), pkg / "__init__.py")
     r = subprocess.run([_sys.executable, "-c", "import bpx"], cwd=str(tmp_path), capture_output=True, text=True)
     assert r.returncode != 0 and "ImportError" in r.stderr and "not built" in r.stderr
+
+
+def test_workload_databases_are_pinned():
+    """Each configured database is the same bytes on every machine and with
+    any thread count: its SHA-256 (lengths, then residues) is pinned."""
+    from bpx import workloads as W
+    q1, d1 = W.config1(3)
+    assert len(q1) == 300 and d1.num_subjects == 10_000 and d1.sha256 == W.SHA256[1]
+    _, d3 = W.config3(8)
+    assert d3.num_subjects == 500_000 and d3.sha256 == W.SHA256[3]
+    del d3
+    qs, d4 = W.config4(8)
+    assert len(qs) == 1024 and d4.num_subjects == 500_000 and d4.sha256 == W.SHA256[4]
+    assert d4.lengths.max() <= 35_000 and 0.15 < (d4.kinds == 2).mean() < 0.25
+    del d4
+    q2, d2 = W.config2(8)
+    assert len(q2) == 500 and d2.num_subjects == 2_000_000 and d2.sha256 == W.SHA256[2]
