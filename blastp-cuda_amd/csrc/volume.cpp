@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -103,7 +104,7 @@ extern "C" {
 
 const char* bpx_volume_last_error(void) { return g_vol_err.c_str(); }
 
-int bpx_volume_read(const char* base, bpx_volume** out) {
+static int bpx_volume_read_impl(const char* base, bpx_volume** out) {
     if (!base || !out) return vfail(BPX_E_ARG, "NULL argument");
     *out = nullptr;
     std::vector<uint8_t> pin, psq;
@@ -126,20 +127,25 @@ int bpx_volume_read(const char* base, bpx_volume** out) {
         return vfail(BPX_E_ARG, b + ".pin: not a version-4 protein index (version " + std::to_string(version) +
                                     ", type " + std::to_string(type) + ")");
     }
-    // header offsets (unused here), then sequence offsets
-    for (uint32_t i = 0; i <= v->n; ++i) c.be32();
-    std::vector<uint32_t> so(v->n + 1);
-    for (uint32_t i = 0; i <= v->n; ++i) so[i] = c.be32();
-    if (!c.ok) {
+    // header offsets (unused here), then sequence offsets: two arrays of
+    // n + 1 big-endian words, which the rest of the file must hold before
+    // anything is sized by n (n = 2^32 - 1 would otherwise wrap n + 1)
+    const uint64_t n1 = (uint64_t)v->n + 1;
+    if (pin.size() < c.at || pin.size() - c.at < 8 * n1) {
         delete v;
-        return vfail(BPX_E_ARG, b + ".pin: truncated offset arrays");
+        return vfail(BPX_E_ARG, b + ".pin: truncated offset arrays (" + std::to_string(v->n) + " sequences)");
     }
-    v->offsets.assign(v->n + 1, 0);
+    c.at += 4 * n1;
+    std::vector<uint32_t> so(n1);
+    for (uint64_t i = 0; i < n1; ++i) so[i] = c.be32();
+    v->offsets.assign(n1, 0);
     v->lengths.assign(v->n, 0);
     uint64_t total = 0;
     uint32_t mx = 0;
     for (uint32_t i = 0; i < v->n; ++i) {
-        if (so[i + 1] < so[i] + 1 || so[i + 1] > psq.size()) {
+        // sequence i is [so[i], so[i+1] - 1) followed by one separator byte;
+        // compared in 64 bits so so[i] = 2^32 - 1 cannot wrap
+        if ((uint64_t)so[i] + 1 > so[i + 1] || so[i + 1] > psq.size()) {
             delete v;
             return vfail(BPX_E_ARG, b + ".psq: sequence " + std::to_string(i) + " outside the file");
         }
@@ -192,7 +198,7 @@ int bpx_volume_destroy(bpx_volume* v) {
     return BPX_OK;
 }
 
-int bpx_volume_write(const char* base, const char* title, const uint8_t* residues, const uint64_t* offsets,
+static int bpx_volume_write_impl(const char* base, const char* title, const uint8_t* residues, const uint64_t* offsets,
                      const uint32_t* lengths, uint32_t n) {
     if (!base || (n && (!residues || !offsets || !lengths))) return vfail(BPX_E_ARG, "NULL argument");
     std::vector<uint8_t> psq{0}, pin, phr;
@@ -237,7 +243,7 @@ int bpx_volume_write(const char* base, const char* title, const uint8_t* residue
 // are taken relative to the alias file's directory), else base itself.
 // names receives them separated by '\n' (capacity bytes including the NUL);
 // *needed gets the required capacity.
-int bpx_db_volumes(const char* base, char* names, size_t capacity, size_t* needed, uint32_t* count) {
+static int bpx_db_volumes_impl(const char* base, char* names, size_t capacity, size_t* needed, uint32_t* count) {
     if (!base) return vfail(BPX_E_ARG, "NULL argument");
     std::vector<std::string> vols;
     const std::string b(base);
@@ -284,7 +290,7 @@ int bpx_db_volumes(const char* base, char* names, size_t capacity, size_t* neede
 // Every volume of a database (alias or single volume), concatenated in
 // DBLIST order, loaded as one resident arena: subject i of the result is
 // the i-th sequence over the volumes (NCBI's OID order).
-int bpx_db_load_volumes(bpx_ctx* ctx, const char* base, bpx_db** out) {
+static int bpx_db_load_volumes_impl(bpx_ctx* ctx, const char* base, bpx_db** out) {
     if (!ctx || !base || !out) return vfail(BPX_E_ARG, "NULL argument");
     size_t need = 0;
     uint32_t nv = 0;
@@ -410,7 +416,7 @@ int parse_gpuinfo(const std::string& path, GpuInfo* gi) {
 
 }  // namespace
 
-int bpx_gpu_info(const char* gpuinfo, uint32_t* num_volumes, uint32_t* group_size, uint32_t* percentage) {
+static int bpx_gpu_info_impl(const char* gpuinfo, uint32_t* num_volumes, uint32_t* group_size, uint32_t* percentage) {
     if (!gpuinfo) return vfail(BPX_E_ARG, "NULL argument");
     GpuInfo gi;
     const int rc = parse_gpuinfo(gpuinfo, &gi);
@@ -421,7 +427,7 @@ int bpx_gpu_info(const char* gpuinfo, uint32_t* num_volumes, uint32_t* group_siz
     return BPX_OK;
 }
 
-int bpx_gpu_volume_read(const char* gpuinfo, uint32_t volume, bpx_volume** out) {
+static int bpx_gpu_volume_read_impl(const char* gpuinfo, uint32_t volume, bpx_volume** out) {
     if (!gpuinfo || !out) return vfail(BPX_E_ARG, "NULL argument");
     *out = nullptr;
     GpuInfo gi;
@@ -475,6 +481,58 @@ int bpx_gpu_volume_read(const char* gpuinfo, uint32_t volume, bpx_volume** out) 
     v->total = v->residues.size();
     *out = v;
     return BPX_OK;
+}
+
+
+// The readers size host arrays from file contents: an allocation failure
+// is reported as BPX_E_NOMEM, never thrown across the C ABI.
+int bpx_volume_read(const char* base, bpx_volume** out) {
+    try {
+        return bpx_volume_read_impl(base, out);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
+}
+
+int bpx_volume_write(const char* base, const char* title, const uint8_t* residues, const uint64_t* offsets,
+                     const uint32_t* lengths, uint32_t n) {
+    try {
+        return bpx_volume_write_impl(base, title, residues, offsets, lengths, n);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
+}
+
+int bpx_db_volumes(const char* base, char* names, size_t capacity, size_t* needed, uint32_t* count) {
+    try {
+        return bpx_db_volumes_impl(base, names, capacity, needed, count);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
+}
+
+int bpx_db_load_volumes(bpx_ctx* ctx, const char* base, bpx_db** out) {
+    try {
+        return bpx_db_load_volumes_impl(ctx, base, out);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
+}
+
+int bpx_gpu_info(const char* gpuinfo, uint32_t* num_volumes, uint32_t* group_size, uint32_t* percentage) {
+    try {
+        return bpx_gpu_info_impl(gpuinfo, num_volumes, group_size, percentage);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
+}
+
+int bpx_gpu_volume_read(const char* gpuinfo, uint32_t volume, bpx_volume** out) {
+    try {
+        return bpx_gpu_volume_read_impl(gpuinfo, volume, out);
+    } catch (const std::bad_alloc&) {
+        return vfail(BPX_E_NOMEM, "host allocation failed");
+    }
 }
 
 }  // extern "C"

@@ -83,3 +83,32 @@ def test_malformed_volumes_are_refused(tmp_path):
     (tmp_path / "e.pal").write_text("TITLE nothing\n")
     with pytest.raises(bpx.BpxError, match="DBLIST"):
         bpx.db_volumes(str(tmp_path / "e"))
+
+
+def test_hostile_index_headers_are_refused(tmp_path):
+    """Header fields that 32-bit arithmetic would wrap: a sequence count of
+    2^32 - 1 (n + 1 = 0) and a first sequence offset of 2^32 - 1 (so[0] + 1 =
+    0).  Both must be refused with a message -- no endless loop, no
+    allocation sized by the count, no read outside the .psq file."""
+    rng = np.random.default_rng(3)
+    res, offs, lens = random_db(rng, 5)
+    base = str(tmp_path / "v")
+    bpx.volume_write(base, res, offs[:-1], lens)
+    pin = bytearray(open(base + ".pin", "rb").read())
+    _, _, title, nseq, _, _ = reference_header_walk(bytes(pin))
+    tlen = len(title)
+    (dlen,) = struct.unpack(">I", bytes(pin[12 + tlen:16 + tlen]))
+    at_n = 16 + tlen + dlen          # sequence count
+    at_so = at_n + 4 + 8 + 4 + 4 * (nseq + 1)  # first sequence offset
+    huge = bytearray(pin)
+    huge[at_n:at_n + 4] = struct.pack(">I", 0xFFFFFFFF)
+    open(str(tmp_path / "h.pin"), "wb").write(huge)
+    os.link(base + ".psq", str(tmp_path / "h.psq"))
+    with pytest.raises(bpx.BpxError, match="truncated"):
+        bpx.volume_read(str(tmp_path / "h"))
+    wrap = bytearray(pin)
+    wrap[at_so:at_so + 4] = struct.pack(">I", 0xFFFFFFFF)
+    open(str(tmp_path / "w.pin"), "wb").write(wrap)
+    os.link(base + ".psq", str(tmp_path / "w.psq"))
+    with pytest.raises(bpx.BpxError, match="outside the file"):
+        bpx.volume_read(str(tmp_path / "w"))
