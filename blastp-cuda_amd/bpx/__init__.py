@@ -526,6 +526,10 @@ def scan_batch(ctx: Context, batch: Batch, db: Database, first: int = 0, count: 
     return Result(h, first, count, flags, batch.num_queries)
 
 
+def last_error() -> str:
+    return _lib.bpx_last_error().decode(errors="replace")
+
+
 def check_memory(ctx: Context, query: Query, db: Database, first=0, count=None, launch=None):
     if count is None:
         count = db.num_subjects - first

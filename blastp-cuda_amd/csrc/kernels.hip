@@ -1048,9 +1048,11 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
             qid = p.qinfo[gi.qinfo_first + r.j].qid;
             dst = p.cnt[(uint64_t)qid * p.count + slot];
         }
-        p.records[dst] = r.h;
-        p.subjects[dst] = p.first + slot;
-        p.queries[dst] = qid;
+        if (dst < p.rec_cap) {
+            p.records[dst] = r.h;
+            p.subjects[dst] = p.first + slot;
+            if (p.queries) p.queries[dst] = qid;
+        }
         ++dst;
     }
 }

@@ -148,7 +148,8 @@ struct CompactParams {
     SegRecord* seg;
     bpx_init_hsp* records;
     uint32_t* subjects;
-    uint32_t* queries;
+    uint32_t* queries;          // optional
+    uint64_t rec_cap;           // entries of records / subjects / queries; records past it are counted, not written
     unsigned long long* total;  // total records
     const uint32_t* overflow;
 };

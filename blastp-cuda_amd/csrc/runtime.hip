@@ -28,15 +28,15 @@ hipError_t launch_features(const uint8_t* residues, const uint64_t* soff, const 
 
 namespace {
 
-thread_local std::string g_err;
+// the last failure's message of each thread (a fixed buffer: reporting a
+// failure allocates nothing)
+thread_local char g_err[512];
 
 int fail(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    std::vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -368,45 +368,69 @@ bpx_launch resolve_launch(const bpx_launch* in) {
     return l;
 }
 
-// Splits the set's groups into launch classes and lays out LDS for each.
-// Subjects [0, nwide) of the processing order (the longest, length + window
-// >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
-int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
-    bpx_ctx* ctx = r->ctx;
-    const bpx_launch L = resolve_launch(user);
-    const bool auto_waves = !user || user->waves_per_group <= 0;
-    const int user_waves = auto_waves ? 0 : std::min(bpx::kMaxWaves, std::max(1, user->waves_per_group));
-    uint32_t maxcell = 0;
-    for (uint32_t c : set.max_cell) maxcell = std::max(maxcell, c);
-    // hit ring: the requested capacity, else 256 entries, shrunk to the
-    // smallest that holds a batch plus the largest cell when LDS is short
-    auto pow2_at_least = [](uint32_t v) {
-        uint32_t n = 1;
-        while (n < v) n <<= 1;
-        return n;
-    };
-    const uint32_t qmin = pow2_at_least(std::max<uint32_t>(128u, maxcell + 64u));
-    const uint32_t qdef = L.queue_capacity > 0 ? std::max(qmin, pow2_at_least((uint32_t)L.queue_capacity))
-                                               : std::max(qmin, 256u);
-    const uint32_t qalt = L.queue_capacity > 0 ? qdef : qmin;
-    const uint32_t sbuf = (uint32_t)L.subject_buffer;
-    const uint32_t limit = (uint32_t)ctx->lds_limit;
-    const bool any_narrow = nwide < count, any_wide = nwide > 0;
+// Largest per-group LDS dimensions of groups that share a launch.
+struct Dims {
+    uint32_t table = 0, qres = 0, nq = 1, diag = 0;
+    void add(const bpx::GroupInfo& g) {
+        table = std::max(table, g.table_words * 4);
+        qres = std::max(qres, g.qres_len);
+        nq = std::max(nq, g.nq);
+        diag = std::max(diag, g.diag_slots);
+    }
+};
 
-    struct Acc {
-        std::vector<uint32_t> groups;
-        uint32_t table = 0, qres = 0, nq = 1, diag = 0;
-        void add(uint32_t gi, const bpx::GroupInfo& g) {
-            groups.push_back(gi);
-            table = std::max(table, g.table_words * 4);
-            qres = std::max(qres, g.qres_len);
-            nq = std::max(nq, g.nq);
-            diag = std::max(diag, g.diag_slots);
-        }
-    };
-    // Per-wave buffers for `waves` waves: the first of (claim table, ring)
-    // = (256, default), (256, small), (128, default), (128, small) that fits.
-    auto plan_of = [&](const Acc& a, bool tab_lds, int waves, bool narrow) {
+// every wave count from 16 down to 8: a packed group's per-wave diagonal
+// tables often fit 13-15 waves (measured 2 % faster on the 1,024-query batch
+// than rounding down to 12)
+constexpr int kAutoWaves[] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 6, 4, 2, 1};
+constexpr int kLdsMinWaves = 8;
+
+// Groups are binned (one launch per bin) by coarse wave counts {16, 12, 8,
+// 4, 2, 1}; a bin then runs at the most waves in its range that all its
+// members fit.  Binning by exact wave count made more launches, each with
+// its own tail: 9 % slower on the skewed 500,000-subject batch.
+int wave_bucket(int w) { return w >= 16 ? 16 : w >= 12 ? 12 : w >= 8 ? 8 : w >= 4 ? 4 : w >= 2 ? 2 : w; }
+int bucket_top(int b) { return b >= 16 ? 16 : b == 12 ? 15 : b == 8 ? 11 : b == 4 ? 7 : b == 2 ? 3 : b; }
+
+// The launch-shape decisions of a scan of one query set over `count`
+// subjects, the first `nwide` of which need wide diagonal slots.  Holds
+// scalars only, so the allocation-free scan plans with it too.
+struct Planner {
+    bpx_ctx* ctx = nullptr;
+    bpx_launch L{};
+    int user_waves = 0;          // 0: automatic
+    uint32_t qdef = 0, qalt = 0;  // hit ring: preferred, and smallest that holds a batch plus the largest cell
+    uint32_t sbuf = 0, limit = 0;
+    bool any_narrow = false, any_wide = false;
+
+    Planner(bpx_ctx* c, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user)
+        : ctx(c), L(resolve_launch(user)) {
+        user_waves = (!user || user->waves_per_group <= 0) ? 0
+                                                           : std::min(bpx::kMaxWaves, std::max(1, user->waves_per_group));
+        uint32_t maxcell = 0;
+        for (uint32_t m : set.max_cell) maxcell = std::max(maxcell, m);
+        // hit ring: the requested capacity, else 256 entries, shrunk to the
+        // smallest that holds a batch plus the largest cell when LDS is short
+        auto pow2_at_least = [](uint32_t v) {
+            uint32_t n = 1;
+            while (n < v) n <<= 1;
+            return n;
+        };
+        const uint32_t qmin = pow2_at_least(std::max<uint32_t>(128u, maxcell + 64u));
+        qdef = L.queue_capacity > 0 ? std::max(qmin, pow2_at_least((uint32_t)L.queue_capacity)) : std::max(qmin, 256u);
+        qalt = L.queue_capacity > 0 ? qdef : qmin;
+        sbuf = (uint32_t)L.subject_buffer;
+        limit = (uint32_t)ctx->lds_limit;
+        any_narrow = nwide < count;
+        any_wide = nwide > 0;
+    }
+    const int* cands() const { return user_waves ? &user_waves : kAutoWaves; }
+    int ncands() const { return user_waves ? 1 : (int)(sizeof kAutoWaves / sizeof kAutoWaves[0]); }
+    int fewest() const { return cands()[ncands() - 1]; }
+
+    // Per-wave buffers for `waves` waves: the first of (claim table, ring) =
+    // (256, default), (256, small), (128, default), (128, small) that fits.
+    bpx::LdsPlan plan_of(const Dims& a, bool tab_lds, int waves, bool narrow) const {
         const uint32_t claims[4] = {256u, 256u, 128u, 128u}, rings[4] = {qdef, qalt, qdef, qalt};
         bpx::LdsPlan pl{};
         for (int k = 0; k < 4; ++k) {
@@ -414,65 +438,102 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             if (pl.total <= limit) break;
         }
         return pl;
-    };
-    // every wave count from 16 down to 8: a packed group's per-wave
-    // diagonal tables often fit 13-15 waves (measured 2 % faster on the
-    // 1,024-query batch than rounding down to 12)
-    static const int kAutoWaves[] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 6, 4, 2, 1};
-    // Groups are binned (one launch per bin) by coarse wave counts {16, 12,
-    // 8, 4, 2, 1}; a bin then runs at the most waves in its range that all
-    // its members fit.  Binning by exact wave count made more launches, each
-    // with its own tail: 9 % slower on the skewed 500,000-subject batch.
-    auto wave_bucket = [](int w) { return w >= 16 ? 16 : w >= 12 ? 12 : w >= 8 ? 8 : w >= 4 ? 4 : w >= 2 ? 2 : w; };
-    auto bucket_top = [](int b) { return b >= 16 ? 16 : b == 12 ? 15 : b == 8 ? 11 : b == 4 ? 7 : b == 2 ? 3 : b; };
-    const int* cands = auto_waves ? kAutoWaves : &user_waves;
-    const int ncands = auto_waves ? (int)(sizeof kAutoWaves / sizeof kAutoWaves[0]) : 1;
-    // the most waves (of the candidates) at which `a` fits for every slot
-    // width this scan needs; 0 if none
-    auto best_waves = [&](const Acc& a, bool tab_lds, int below) {
-        for (int k = 0; k < ncands; ++k) {
-            const int w = cands[k];
+    }
+    // the most waves (of the candidates, at most `below`) at which `a` fits
+    // for every slot width this scan needs; 0 if none
+    int best_waves(const Dims& a, bool tab_lds, int below) const {
+        for (int k = 0; k < ncands(); ++k) {
+            const int w = cands()[k];
             if (w > below) continue;
             if ((!any_narrow || plan_of(a, tab_lds, w, true).total <= limit) &&
                 (!any_wide || plan_of(a, tab_lds, w, false).total <= limit))
                 return w;
         }
         return 0;
+    }
+    // Where one group runs with the most waves: its table in LDS when that
+    // keeps at least kLdsMinWaves waves (or as many as the HBM table would),
+    // else its table in HBM.  waves = 0: it fits nowhere.
+    void place(const bpx::GroupInfo& g, bool* lds, int* waves) const {
+        Dims one;
+        one.add(g);
+        const int wl = (g.flags & bpx::kGroupNarrowStarts) ? best_waves(one, true, bpx::kMaxWaves) : 0;
+        const int wh = best_waves(one, false, bpx::kMaxWaves);
+        *lds = wl > 0 && (wl >= kLdsMinWaves || wl >= wh);
+        *waves = *lds ? wl : wh;
+    }
+    int refuse(const Dims& a, bool tab_lds, int waves, uint32_t* lds_need) const {
+        const bpx::LdsPlan pl = plan_of(a, tab_lds, waves, any_narrow);
+        *lds_need = std::max(*lds_need, pl.total);
+        return fail(BPX_E_BUDGET,
+                    "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
+                    "use fewer waves per group or a smaller subject buffer",
+                    pl.total, limit, a.diag, waves, sbuf);
+    }
+    // A launch class over subjects [lo, hi) of the processing order; the
+    // caller sets its items.
+    bpx_result::Class make_class(const Dims& a, bool tab_lds, bool multi, bool narrow, int waves, uint32_t lo,
+                                 uint32_t hi) const {
+        bpx_result::Class c;
+        c.tab_lds = tab_lds;
+        c.multi = multi;
+        c.narrow = narrow;
+        c.waves = waves;
+        c.plan = plan_of(a, tab_lds, waves, narrow);
+        c.diag_cap = std::max<uint32_t>(a.diag, 1);
+        if (L.subjects_per_item <= 0) {
+            // one item per group; every workgroup works through them in order
+            // and its waves claim subjects device-wide, a few at a time: about
+            // 1/16 of a wave's share (at most kFlatChunk), so the longest
+            // subjects, which go first, cannot pile up on a few waves, and the
+            // counters see few enough atomics.  256 per-query scans of 500,000
+            // subjects took 0.80 s at 8 at a time, 0.84 s at 16 and 1.35 s at
+            // 32; on 2,000,000 subjects 8, 16 and 32 are equal and 1 is 2.3x
+            // slower (one atomic per subject).
+            const uint64_t wv = (uint64_t)ctx->num_cus * L.groups_per_cu * waves;
+            c.flat = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kFlatChunk, (hi - lo) / (wv * 16)));
+        }
+        return c;
+    }
+};
+
+// Splits the set's groups into launch classes and lays out LDS for each.
+// Subjects [0, nwide) of the processing order (the longest, length + window
+// >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
+int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
+    const Planner P(r->ctx, set, count, nwide, user);
+    struct Acc {
+        std::vector<uint32_t> groups;
+        Dims d;
     };
-    // Bin the groups: each goes where it runs with the most waves -- its
-    // table in LDS when that keeps at least kLdsMinWaves waves (or as many as
-    // the HBM table would), else its table in HBM.  Groups of one bin share
-    // a launch.  Key: (table in LDS, multi-query, waves).
-    constexpr int kLdsMinWaves = 8;
+    // Bin the groups: each goes where it runs with the most waves.  Groups of
+    // one bin share a launch.  Key: (table in LDS, multi-query, waves).
     std::map<std::tuple<int, int, int>, Acc> bins;
     for (uint32_t gi = 0; gi < set.groups.size(); ++gi) {
         const bpx::GroupInfo& g = set.groups[gi];
-        Acc one;
-        one.add(gi, g);
-        const int wl = (g.flags & bpx::kGroupNarrowStarts) ? best_waves(one, true, bpx::kMaxWaves) : 0;
-        const int wh = best_waves(one, false, bpx::kMaxWaves);
-        if (wl == 0 && wh == 0) {
-            const bpx::LdsPlan pl = plan_of(one, false, cands[ncands - 1], any_narrow);
-            r->lds_need = std::max(r->lds_need, pl.total);
-            return fail(BPX_E_BUDGET,
-                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
-                        "use fewer waves per group or a smaller subject buffer",
-                        pl.total, limit, one.diag, cands[ncands - 1], sbuf);
+        bool lds = false;
+        int w = 0;
+        P.place(g, &lds, &w);
+        if (w == 0) {
+            Dims one;
+            one.add(g);
+            return P.refuse(one, false, P.fewest(), &r->lds_need);
         }
-        const bool lds = wl > 0 && (wl >= kLdsMinWaves || wl >= wh);
-        bins[std::make_tuple(lds ? 1 : 0, g.nq > 1 ? 1 : 0, wave_bucket(lds ? wl : wh))].add(gi, g);
+        Acc& a = bins[std::make_tuple(lds ? 1 : 0, g.nq > 1 ? 1 : 0, wave_bucket(w))];
+        a.groups.push_back(gi);
+        a.d.add(g);
     }
     r->classes.clear();
     r->items.clear();
     // By default each launch class has one work item per group, and its
-    // subjects are claimed device-wide by the wavefronts (below): no
-    // workgroup barrier between items, so no wave waits for the slowest wave
-    // of an item.  Runs of 8 items per CU (barriers between them) took 11.2
-    // ms on the 2,000,000-subject scan against 10.6 ms; per-query scans of a
-    // 500,000-subject database took 12 % longer, a 1,024-query batch 2 %.
+    // subjects are claimed device-wide by the wavefronts (Planner::make_class):
+    // no workgroup barrier between items, so no wave waits for the slowest
+    // wave of an item.  Runs of 8 items per CU (barriers between them) took
+    // 11.2 ms on the 2,000,000-subject scan against 10.6 ms; per-query scans
+    // of a 500,000-subject database took 12 % longer, a 1,024-query batch 2 %.
     // An explicit subjects_per_item keeps runs of that many subjects, each
     // claimed by one workgroup.
-    const uint32_t R = (uint32_t)std::max(1, L.subjects_per_item);
+    const uint32_t R = (uint32_t)std::max(1, P.L.subjects_per_item);
     // LDS bins first, then by waves, most first
     std::vector<std::pair<std::tuple<int, int, int>, Acc*>> order;
     for (auto& kv : bins) order.push_back({kv.first, &kv.second});
@@ -484,45 +545,21 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
     for (const auto& ob : order) {
         const Acc& a = *ob.second;
         const bool tab_lds = std::get<0>(ob.first) != 0;
-        if (a.diag > 65536u)
+        if (a.d.diag > 65536u)
             return fail(BPX_E_BUDGET, "a scan group needs %u diagonal slots (at most 65536); use smaller query groups",
-                        a.diag);
+                        a.d.diag);
         // the bin's largest members together may need fewer waves than each alone
-        const int waves = best_waves(a, tab_lds, bucket_top(std::get<2>(ob.first)));
-        if (waves == 0) {
-            const bpx::LdsPlan pl = plan_of(a, tab_lds, std::get<2>(ob.first), any_narrow);
-            r->lds_need = std::max(r->lds_need, pl.total);
-            return fail(BPX_E_BUDGET,
-                        "LDS plan %u B exceeds %u B (diagonal slots %u, %d waves, subject buffer %u); "
-                        "use fewer waves per group or a smaller subject buffer",
-                        pl.total, limit, a.diag, std::get<2>(ob.first), sbuf);
-        }
+        const int waves = P.best_waves(a.d, tab_lds, bucket_top(std::get<2>(ob.first)));
+        if (waves == 0) return P.refuse(a.d, tab_lds, std::get<2>(ob.first), &r->lds_need);
         for (int w = 0; w < 2; ++w) {
             const bool narrow = w == 1;
             const uint32_t lo = narrow ? nwide : 0u, hi = narrow ? count : nwide;
             if (lo >= hi) continue;
             if ((int)r->classes.size() >= kMaxClasses) return fail(BPX_E_BUDGET, "too many launch classes");
-            bpx_result::Class c;
-            c.tab_lds = tab_lds;
-            c.multi = std::get<1>(ob.first) != 0;
-            c.narrow = narrow;
-            c.waves = waves;
-            c.plan = plan_of(a, tab_lds, waves, narrow);
+            bpx_result::Class c = P.make_class(a.d, tab_lds, std::get<1>(ob.first) != 0, narrow, waves, lo, hi);
             r->lds_need = std::max(r->lds_need, c.plan.total);
-            c.diag_cap = std::max<uint32_t>(a.diag, 1);
             c.item_first = (uint32_t)r->items.size();
-            if (L.subjects_per_item <= 0) {
-                // one item per group; every workgroup works through them in
-                // order and its waves claim subjects device-wide, a few at a
-                // time: about 1/16 of a wave's share (at most kFlatChunk), so
-                // the longest subjects, which go first, cannot pile up on a
-                // few waves, and the counters see few enough atomics.  256
-                // per-query scans of 500,000 subjects took 0.80 s at 8 at a
-                // time, 0.84 s at 16 and 1.35 s at 32; on 2,000,000 subjects
-                // 8, 16 and 32 are equal and 1 is 2.3x slower (one atomic per
-                // subject).
-                const uint64_t wv = (uint64_t)ctx->num_cus * L.groups_per_cu * waves;
-                c.flat = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kFlatChunk, (hi - lo) / (wv * 16)));
+            if (c.flat) {
                 for (uint32_t gi : a.groups) r->items.push_back({gi, lo, hi - lo, 0});
             } else {
                 for (uint32_t gi : a.groups)
@@ -532,7 +569,7 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             r->classes.push_back(c);
         }
     }
-    r->launch = L;
+    r->launch = P.L;
     return BPX_OK;
 }
 
@@ -628,47 +665,77 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     return BPX_OK;
 }
 
-// The scan's device work on the context stream: counter resets (fill
-// kernels, no memset nodes), every launch class, compaction and the copy of
-// the meta words to pinned host memory (the scan kernel writes the subject
-// counters to host memory itself).  `events` brackets
-// the word-finder launches with the timing events (not when captured into a
-// graph: then the caller brackets the graph launch).  The work items must be
-// on the device already (upload_items).
-int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start_event = true) {
-    bpx_ctx* ctx = r->ctx;
-    const QuerySet& set = *r->set;
-    const bpx_db* db = r->db;
-    hipStream_t st = r->stream;
+// Everything the device work of one scan reads and writes: the result's
+// pooled buffers, or the allocation-free scan's scratch.
+struct ScanJob {
+    bpx_ctx* ctx = nullptr;
+    const QuerySet* set = nullptr;
+    const bpx_db* db = nullptr;
+    uint32_t first = 0, count = 0;
+    const bpx_result::Class* classes = nullptr;
+    size_t nclasses = 0, nitems = 0;
+    int groups_per_cu = 1;
+    uint32_t sbuf = 0;  // subject buffer bytes per wave
+    hipStream_t stream = nullptr;
+    const uint32_t* order = nullptr;  // processing order and descriptors in that order
+    const uint64_t* poff = nullptr;
+    const uint32_t* plen = nullptr;
+    uint32_t* cnt = nullptr;
+    int32_t* sstats = nullptr;  // optional per-subject counters
+    uint32_t* run_loc = nullptr;
+    uint32_t* run_len = nullptr;
+    bpx::SegRecord* seg = nullptr;
+    uint32_t seg_cap = 0;
+    uint32_t* tile = nullptr;
+    bpx_init_hsp* records = nullptr;
+    uint32_t* subjects = nullptr;
+    uint32_t* queries = nullptr;  // optional
+    uint64_t rec_cap = ~0ull;     // records / subjects / queries capacity
+    unsigned long long* meta = nullptr;
+    bpx::WorkItem* items = nullptr;  // on the device
+    uint32_t* work = nullptr;
+    unsigned long long* h_meta = nullptr;  // host copy of the meta words
+    hipEvent_t ev_start = nullptr, ev_kernels = nullptr;  // timing, optional
+    bool capturing = false;  // the stream is being captured: kernel attributes are set beforehand
+};
+
+// The scan's device work on the job's stream: counter resets (fill kernels,
+// no memset nodes), every launch class, compaction and the copy of the meta
+// words to host memory (the scan kernel writes the subject counters itself).
+// The events bracket the word-finder launches when given.  The work items
+// must be on the device already.
+int enqueue_job(const ScanJob& j) {
+    const QuerySet& set = *j.set;
+    hipStream_t st = j.stream;
     // meta words, work counters, and the counts and run lengths, which the
     // kernel writes only for subjects with records: one launch
     bpx::ZeroRanges z{};
-    z.dst[0] = reinterpret_cast<uint32_t*>(r->d_meta);
+    z.dst[0] = reinterpret_cast<uint32_t*>(j.meta);
     z.n[0] = 2 * (kMetaWords + 5ull * set.nq);
-    z.dst[1] = r->d_work;
-    z.n[1] = kMaxClasses + r->items.size();
-    z.dst[2] = r->d_cnt;
-    z.n[2] = (uint64_t)set.nq * r->count;
-    z.dst[3] = r->d_run_len;
-    z.n[3] = (uint64_t)set.groups.size() * r->count;
+    z.dst[1] = j.work;
+    z.n[1] = kMaxClasses + j.nitems;
+    z.dst[2] = j.cnt;
+    z.n[2] = (uint64_t)set.nq * j.count;
+    z.dst[3] = j.run_len;
+    z.n[3] = (uint64_t)set.groups.size() * j.count;
     z.count = 4;
     HIP_TRY(bpx::launch_zero_u32(z, st));
-    unsigned long long* ustats = r->d_meta + kMetaWords;
-    uint32_t* overflow = reinterpret_cast<uint32_t*>(r->d_meta + 1);
-    if (events && start_event) HIP_TRY(hipEventRecord(r->ev[0], st));
+    unsigned long long* ustats = j.meta + kMetaWords;
+    uint32_t* overflow = reinterpret_cast<uint32_t*>(j.meta + 1);
+    if (j.ev_start) HIP_TRY(hipEventRecord(j.ev_start, st));
     uint64_t seg_at = 0;
-    for (size_t ci = 0; ci < r->classes.size(); ++ci) {
-        const auto& c = r->classes[ci];
-        const int groups = ctx->num_cus * r->launch.groups_per_cu;
+    const int groups = j.ctx->num_cus * j.groups_per_cu;
+    for (size_t ci = 0; ci < j.nclasses; ++ci) {
+        const auto& c = j.classes[ci];
         bpx::ScanParams sp{};
-        sp.residues = db->d_arena;
-        sp.soff = db->d_off;
-        sp.slen = db->d_len;
-        sp.order = order;
-        sp.poff = r->d_poff ? r->d_poff : db->d_poff;
-        sp.plen = r->d_plen ? r->d_plen : db->d_plen;
-        sp.first = r->first;
-        sp.count = r->count;
+        sp.residues = j.db->d_arena;
+        sp.soff = j.db->d_off;
+        sp.slen = j.db->d_len;
+        sp.order = j.order;
+        sp.poff = j.poff;
+        sp.plen = j.plen;
+        sp.first = j.first;
+        sp.count = j.count;
         sp.groups = set.d_groups;
         sp.qinfo = set.d_qinfo;
         sp.tables = set.d_tables;
@@ -677,24 +744,24 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
         sp.ngroups_pv = set.ngroups_pv;
         sp.word = set.word;
         sp.window = set.window;
-        sp.items = r->d_items + c.item_first;
+        sp.items = j.items + c.item_first;
         sp.nitems = c.nitems;
         sp.flat = c.flat;
-        sp.flat_counters = r->d_work + kMaxClasses + c.item_first;
-        sp.work_counter = r->d_work + ci;
-        sp.cnt = r->d_cnt;
-        sp.stats = r->d_sstats;
-        sp.run_loc = r->d_run_loc;
-        sp.run_len = r->d_run_len;
+        sp.flat_counters = j.work + kMaxClasses + c.item_first;
+        sp.work_counter = j.work + ci;
+        sp.cnt = j.cnt;
+        sp.stats = j.sstats;
+        sp.run_loc = j.run_loc;
+        sp.run_len = j.run_len;
         sp.ustats = ustats;
         // each launch class owns its own range of wave segments; run
         // locations written by the kernel are relative to sp.seg, so they
         // carry the class offset below
-        sp.seg = r->d_seg;
-        sp.seg_cap = r->seg_cap;
-        sp.seg_wave_base = (uint32_t)(seg_at / r->seg_cap);
+        sp.seg = j.seg;
+        sp.seg_cap = j.seg_cap;
+        sp.seg_wave_base = (uint32_t)(seg_at / j.seg_cap);
         sp.overflow = overflow;
-        sp.max_fill = reinterpret_cast<uint32_t*>(r->d_meta + 2);
+        sp.max_fill = reinterpret_cast<uint32_t*>(j.meta + 2);
         sp.lds_table = c.plan.table;
         sp.lds_qres = c.plan.qres;
         sp.lds_qmap = c.plan.qmap;
@@ -709,31 +776,73 @@ int enqueue_device(bpx_result* r, const uint32_t* order, bool events, bool start
         sp.diag_cap = c.diag_cap;
         sp.qcap = c.plan.qcap;
         sp.claim_slots = c.plan.claim_slots;
-        sp.sbuf = (uint32_t)r->launch.subject_buffer;
-        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st, events));
-        seg_at += (uint64_t)groups * c.waves * r->seg_cap;
+        sp.sbuf = j.sbuf;
+        HIP_TRY(bpx::launch_scan(sp, c.tab_lds, c.multi, c.narrow, c.waves, groups, st, !j.capturing));
+        seg_at += (uint64_t)groups * c.waves * j.seg_cap;
     }
-    if (events) HIP_TRY(hipEventRecord(r->ev[1], st));
+    if (j.ev_kernels) HIP_TRY(hipEventRecord(j.ev_kernels, st));
     bpx::CompactParams cp{};
-    cp.cnt = r->d_cnt;
-    cp.n = (uint64_t)set.nq * r->count;
-    cp.tile_sum = r->d_tile;
-    cp.run_loc = r->d_run_loc;
-    cp.run_len = r->d_run_len;
+    cp.cnt = j.cnt;
+    cp.n = (uint64_t)set.nq * j.count;
+    cp.tile_sum = j.tile;
+    cp.run_loc = j.run_loc;
+    cp.run_len = j.run_len;
     cp.groups = set.d_groups;
     cp.qinfo = set.d_qinfo;
     cp.ngroups = (uint32_t)set.groups.size();
-    cp.count = r->count;
-    cp.first = r->first;
-    cp.seg = r->d_seg;
-    cp.records = r->d_records;
-    cp.subjects = r->d_subjects;
-    cp.queries = r->d_queries;
-    cp.total = r->d_meta;
+    cp.count = j.count;
+    cp.first = j.first;
+    cp.seg = j.seg;
+    cp.records = j.records;
+    cp.subjects = j.subjects;
+    cp.queries = j.queries;
+    cp.rec_cap = j.rec_cap;
+    cp.total = j.meta;
     cp.overflow = overflow;
     HIP_TRY(bpx::launch_compact(cp, st));
-    HIP_TRY(hipMemcpyAsync(r->h_meta, r->d_meta, (kMetaWords + 5ull * set.nq) * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(j.h_meta, j.meta, (kMetaWords + 5ull * set.nq) * 8, hipMemcpyDeviceToHost, st));
     return BPX_OK;
+}
+
+// The job of a pooled result.  `events` brackets the word-finder launches
+// with the result's timing events (not when captured into a graph: then the
+// caller brackets the graph launch).
+ScanJob result_job(bpx_result* r, const uint32_t* order, bool events, bool start_event) {
+    ScanJob j;
+    j.ctx = r->ctx;
+    j.set = r->set;
+    j.db = r->db;
+    j.first = r->first;
+    j.count = r->count;
+    j.classes = r->classes.data();
+    j.nclasses = r->classes.size();
+    j.nitems = r->items.size();
+    j.groups_per_cu = r->launch.groups_per_cu;
+    j.sbuf = (uint32_t)r->launch.subject_buffer;
+    j.stream = r->stream;
+    j.order = order;
+    j.poff = r->d_poff ? r->d_poff : r->db->d_poff;
+    j.plen = r->d_plen ? r->d_plen : r->db->d_plen;
+    j.cnt = r->d_cnt;
+    j.sstats = r->d_sstats;
+    j.run_loc = r->d_run_loc;
+    j.run_len = r->d_run_len;
+    j.seg = r->d_seg;
+    j.seg_cap = r->seg_cap;
+    j.tile = r->d_tile;
+    j.records = r->d_records;
+    j.subjects = r->d_subjects;
+    j.queries = r->d_queries;
+    j.meta = r->d_meta;
+    j.items = r->d_items;
+    j.work = r->d_work;
+    j.h_meta = r->h_meta;
+    if (events) {
+        if (start_event) j.ev_start = r->ev[0];
+        j.ev_kernels = r->ev[1];
+    }
+    j.capturing = !events;
+    return j;
 }
 
 int upload_items(bpx_result* r) {
@@ -747,7 +856,7 @@ int upload_items(bpx_result* r) {
 // then includes the launch whose segments overflowed
 int enqueue(bpx_result* r, const uint32_t* order, bool start_event = true) {
     int rc = upload_items(r);
-    return rc ? rc : enqueue_device(r, order, true, start_event);
+    return rc ? rc : enqueue_job(result_job(r, order, true, start_event));
 }
 
 void graph_release(bpx_result* r) {
@@ -774,7 +883,7 @@ int graph_capture_and_launch(bpx_result* r) {
     }
     HIP_TRY(hipStreamSynchronize(r->stream));
     HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_device(r, order, false);
+    rc = enqueue_job(result_job(r, order, false, false));
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(r->stream, &g);
     if (rc) {
@@ -1051,9 +1160,13 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
 }
 }  // namespace
 
+namespace bpx {
+void set_last_error(const char* msg) { std::snprintf(g_err, sizeof g_err, "%s", msg ? msg : ""); }
+}  // namespace bpx
+
 extern "C" {
 
-const char* bpx_last_error(void) { return g_err.c_str(); }
+const char* bpx_last_error(void) { return g_err; }
 
 const int8_t* bpx_blosum62(void) { return bpx::blosum62(); }
 

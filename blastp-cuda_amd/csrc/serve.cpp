@@ -17,11 +17,17 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "bpx.h"
+
+namespace bpx {
+// sets bpx_last_error() of the calling thread (runtime.hip)
+void set_last_error(const char* msg);
+}
 
 namespace {
 
@@ -33,6 +39,7 @@ struct Request {
     Clock::time_point t_submit, t_launch, t_done;
     bool done = false;
     int status = BPX_OK;
+    std::string err;  // bpx_last_error() of the worker when status != BPX_OK
     uint32_t batch_size = 0;
     std::vector<bpx_init_hsp> recs;
     std::vector<uint32_t> subs;
@@ -85,6 +92,7 @@ void bpx_server::finish(Inflight& f) {
         rc = bpx_result_records(f.r, recs.data(), subs.data(), qids.data(), recs.size());
     }
     if (rc == BPX_OK) rc = bpx_result_query_stats(f.r, qst.data(), (uint32_t)qst.size());
+    const std::string err = rc == BPX_OK ? std::string() : std::string(bpx_last_error());
     const auto now = Clock::now();
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -99,6 +107,7 @@ void bpx_server::finish(Inflight& f) {
                 q.stats = qst[j];
             }
             q.status = rc;
+            q.err = err;
             q.t_done = now;
             q.done = true;
         }
@@ -152,21 +161,27 @@ void bpx_server::run() {
             if (rc == BPX_OK) rc = bpx_scan_batch(ctx, f.b, db, 0, nsub, &cfg.launch, 0u, &f.r);
             const auto now = Clock::now();
             if (rc != BPX_OK) {
+                const std::string err = bpx_last_error();
                 bpx_batch_destroy(f.b);
                 std::lock_guard<std::mutex> lk(mu);
                 for (auto& q : take) {
                     q->status = rc;
+                    q->err = err;
                     q->t_launch = q->t_done = now;
                     q->done = true;
                 }
                 cv_done.notify_all();
                 continue;
             }
-            for (auto& q : take) {
-                q->t_launch = now;
-                q->batch_size = (uint32_t)take.size();
+            {
+                // batches is read by bpx_server_counters on other threads
+                std::lock_guard<std::mutex> lk(mu);
+                for (auto& q : take) {
+                    q->t_launch = now;
+                    q->batch_size = (uint32_t)take.size();
+                }
+                ++batches;
             }
-            ++batches;
             fl.push_back(std::move(f));
             continue;
         }
@@ -258,6 +273,8 @@ int bpx_server_wait(bpx_server* s, uint64_t ticket, int32_t timeout_ms, bpx_requ
         info->queue_ms = ms_between(r->t_submit, r->t_launch);
         info->batch_size = r->batch_size;
     }
+    // a failure of the request's batch is reported on the caller's thread
+    if (r->status != BPX_OK) bpx::set_last_error(r->err.c_str());
     return r->status;
 }
 
