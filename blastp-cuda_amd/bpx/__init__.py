@@ -125,6 +125,8 @@ _sig("bpx_check_memory", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINT
      C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("bpx_scan", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
      C.POINTER(_vp))
+_sig("bpx_scan_noalloc", C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, C.POINTER(C.c_uint64),
+     C.POINTER(UngappedStats))
 _sig("bpx_result_wait", C.c_int, _vp)
 _sig("bpx_result_timing", C.c_int, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _sig("bpx_result_summary", C.c_int, _vp, C.POINTER(C.c_uint64), C.POINTER(UngappedStats))
@@ -524,6 +526,23 @@ def scan_batch(ctx: Context, batch: Batch, db: Database, first: int = 0, count: 
     _check(_lib.bpx_scan_batch(ctx._h, batch._h, db._h, first, count, C.byref(lp) if lp is not None else None,
                                flags, C.byref(h)))
     return Result(h, first, count, flags, batch.num_queries)
+
+
+NOALLOC_MAX_SUBJECTS = 2097152
+NOALLOC_SEG_RECORDS = 1048576
+
+
+def scan_noalloc(ctx: Context, query: Query, db: Database, d_records: int, d_subjects: int, capacity: int,
+                 d_stats: int | None = None):
+    """bpx_scan_noalloc: a synchronous scan of the whole database that
+    allocates nothing; the outputs are caller-owned device buffers given as
+    addresses.  Returns (status, number of records, ungapped stats); a
+    status other than 0 leaves the reason in last_error()."""
+    n = C.c_uint64()
+    st = UngappedStats()
+    rc = _lib.bpx_scan_noalloc(ctx._h, query._h, db._h, d_records or None, d_subjects or None, capacity,
+                               d_stats or None, C.byref(n), C.byref(st))
+    return rc, int(n.value), st.as_dict()
 
 
 def last_error() -> str:

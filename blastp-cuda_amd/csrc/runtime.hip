@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <numeric>
 #include <string>
@@ -128,6 +129,10 @@ struct bpx_ctx {
     // counters the scan kernel writes to directly: coherent, so its stores
     // go to host memory as they are made, not to the device caches
     Pool coherent{hipHostMallocMapped | hipHostMallocCoherent};
+    // this device's copy of the allocation-free scan's scratch array
+    // (hipGetSymbolAddress at context creation), or null with the reason
+    uint8_t* scratch = nullptr;
+    char scratch_err[256] = {0};
 };
 
 struct bpx_db {
@@ -1160,6 +1165,54 @@ int upload_arena(bpx_ctx* ctx, uint8_t* d_arena, const uint8_t* residues, const 
 }
 }  // namespace
 
+// ---- the allocation-free scan's scratch ------------------------------------------
+//
+// One file-scope device array, a copy per GPU, holds every buffer of
+// bpx_scan_noalloc beyond its inputs and outputs, at fixed offsets.  Host
+// code never uses the array's name as an address: each context keeps its
+// device's address from hipGetSymbolAddress.
+namespace {
+constexpr uint64_t kNaMeta = 0;                                         // kMetaWords + 5 u64
+constexpr uint64_t kNaWork = 256;                                       // kMaxClasses + 2 u32
+constexpr uint64_t kNaItems = kNaWork + 1024;                           // 2 WorkItem
+constexpr uint64_t kNaTile = kNaItems + 256;                            // tile sums
+constexpr uint64_t kNaCnt = kNaTile + BPX_NOALLOC_MAX_SUBJECTS / bpx::kTile * 4ull;
+constexpr uint64_t kNaRunLoc = kNaCnt + BPX_NOALLOC_MAX_SUBJECTS * 4ull;
+constexpr uint64_t kNaRunLen = kNaRunLoc + BPX_NOALLOC_MAX_SUBJECTS * 4ull;
+constexpr uint64_t kNaSeg = kNaRunLen + BPX_NOALLOC_MAX_SUBJECTS * 4ull;
+constexpr uint64_t kNaBytes = kNaSeg + BPX_NOALLOC_SEG_RECORDS * (uint64_t)sizeof(bpx::SegRecord);
+static_assert((kMetaWords + 5) * 8 <= kNaWork && (kMaxClasses + 2) * 4 <= kNaItems - kNaWork &&
+                  2 * sizeof(bpx::WorkItem) <= kNaTile - kNaItems,
+              "scratch regions overlap");
+static_assert(kNaCnt % 256 == 0 && kNaSeg % 256 == 0, "scratch regions are 256-byte aligned");
+static_assert(kNaBytes == BPX_NOALLOC_SCRATCH_BYTES, "BPX_NOALLOC_SCRATCH_BYTES is the scratch size");
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+// The scratch array's address on `device`, which must be current: resolved
+// once per device, then reused by every context on it.
+hipError_t scratch_address(int device, uint8_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch[device] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+}  // namespace
+
 namespace bpx {
 void set_last_error(const char* msg) { std::snprintf(g_err, sizeof g_err, "%s", msg ? msg : ""); }
 }  // namespace bpx
@@ -1199,6 +1252,12 @@ int bpx_context_create(int device, bpx_ctx** out) {
         delete ctx;
         return fail(BPX_E_HIP, "stream / counter allocation failed");
     }
+    // the allocation-free scan's scratch on this device (current); a failure
+    // is reported by bpx_scan_noalloc, the only user
+    e = scratch_address(device, &ctx->scratch);
+    if (e != hipSuccess)
+        std::snprintf(ctx->scratch_err, sizeof ctx->scratch_err, "hipGetSymbolAddress(scratch) on device %d: %s",
+                      device, hipGetErrorString(e));
     *out = ctx;
     return BPX_OK;
 }
@@ -1490,6 +1549,110 @@ int bpx_scan_batch(bpx_ctx* ctx, const bpx_batch* b, const bpx_db* db, uint32_t 
                    const bpx_launch* launch, uint32_t out_flags, bpx_result** out) {
     if (!b) return fail(BPX_E_ARG, "batch is NULL");
     return start_scan(ctx, b->set, db, first, count, launch, out_flags, out);
+}
+
+int bpx_scan_noalloc(bpx_ctx* ctx, const bpx_query* q, const bpx_db* db, bpx_init_hsp* d_records,
+                     uint32_t* d_subjects, uint64_t record_capacity, bpx_subject_stats* d_stats,
+                     uint64_t* num_records, bpx_ungapped_stats* stats) {
+    if (!ctx || !q || !db || !num_records) return fail(BPX_E_ARG, "NULL argument");
+    *num_records = 0;
+    if (record_capacity && (!d_records || !d_subjects)) return fail(BPX_E_ARG, "record buffers are NULL");
+    const QuerySet& set = q->set;
+    if (set.ctx != ctx || db->ctx != ctx) return fail(BPX_E_ARG, "handles belong to another context");
+    if (set.groups.size() != 1 || set.nq != 1) return fail(BPX_E_ARG, "one prepared query expected");
+    if (!ctx->scratch) return fail(BPX_E_HIP, "no scratch: %s", ctx->scratch_err);
+    if (db->n > BPX_NOALLOC_MAX_SUBJECTS)
+        return fail(BPX_E_BUDGET, "%u subjects exceed the allocation-free scan's limit of %u; use bpx_scan", db->n,
+                    BPX_NOALLOC_MAX_SUBJECTS);
+    if (db->cost_order && set.window > kCostMaxWindow)
+        return fail(BPX_E_ARG, "window %d above %d while the database has a cost order", set.window, kCostMaxWindow);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // the launch plan of bpx_scan for one query: a wide-slot and a
+    // narrow-slot class at most, one work item each, in fixed arrays
+    const uint32_t count = db->n, nwide = count_wide(db, 0, count, set.window);
+    const Planner P(ctx, set, count, nwide, nullptr);
+    bool lds = false;
+    int waves = 0;
+    P.place(set.groups[0], &lds, &waves);
+    Dims d;
+    d.add(set.groups[0]);
+    uint32_t lds_need = 0;
+    if (waves == 0) return P.refuse(d, false, P.fewest(), &lds_need);
+    bpx_result::Class cls[2];
+    bpx::WorkItem items[2] = {};
+    uint32_t nc = 0;
+    uint64_t nwaves = 0;
+    for (int w = 0; w < 2; ++w) {
+        const bool narrow = w == 1;
+        const uint32_t lo = narrow ? nwide : 0u, hi = narrow ? count : nwide;
+        if (lo >= hi) continue;
+        cls[nc] = P.make_class(d, lds, false, narrow, waves, lo, hi);
+        cls[nc].item_first = nc;
+        cls[nc].nitems = 1;
+        items[nc] = bpx::WorkItem{0u, lo, hi - lo, 0u};
+        nwaves += (uint64_t)ctx->num_cus * P.L.groups_per_cu * waves;
+        ++nc;
+    }
+    const uint64_t seg_cap = nwaves ? BPX_NOALLOC_SEG_RECORDS / nwaves : 1;
+    if (seg_cap < 16)
+        return fail(BPX_E_BUDGET, "%llu wavefronts leave fewer than 16 record slots each in the scratch",
+                    (unsigned long long)nwaves);
+    uint8_t* S = ctx->scratch;
+    unsigned long long h_meta[kMetaWords + 5] = {};
+    ScanJob j;
+    j.ctx = ctx;
+    j.set = &set;
+    j.db = db;
+    j.first = 0;
+    j.count = count;
+    j.classes = cls;
+    j.nclasses = nc;
+    j.nitems = nc;
+    j.groups_per_cu = P.L.groups_per_cu;
+    j.sbuf = (uint32_t)P.L.subject_buffer;
+    j.stream = ctx->stream;
+    j.order = db->d_order;
+    j.poff = db->d_poff;
+    j.plen = db->d_plen;
+    j.cnt = reinterpret_cast<uint32_t*>(S + kNaCnt);
+    j.sstats = reinterpret_cast<int32_t*>(d_stats);
+    j.run_loc = reinterpret_cast<uint32_t*>(S + kNaRunLoc);
+    j.run_len = reinterpret_cast<uint32_t*>(S + kNaRunLen);
+    j.seg = reinterpret_cast<bpx::SegRecord*>(S + kNaSeg);
+    j.seg_cap = (uint32_t)seg_cap;
+    j.tile = reinterpret_cast<uint32_t*>(S + kNaTile);
+    j.records = d_records;
+    j.subjects = d_subjects;
+    j.queries = nullptr;
+    j.rec_cap = record_capacity;
+    j.meta = reinterpret_cast<unsigned long long*>(S + kNaMeta);
+    j.items = reinterpret_cast<bpx::WorkItem*>(S + kNaItems);
+    j.work = reinterpret_cast<uint32_t*>(S + kNaWork);
+    j.h_meta = h_meta;
+    if (nc) HIP_TRY(hipMemcpyAsync(j.items, items, nc * sizeof(bpx::WorkItem), hipMemcpyHostToDevice, ctx->stream));
+    const int rc = enqueue_job(j);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(BPX_E_HIP, "allocation-free scan: %s", hipGetErrorString(e));
+    const uint32_t ovf = (uint32_t)(h_meta[1] & 0xFFFFFFFFu);
+    if (ovf & 2u) return fail(BPX_E_STATE, "internal: hit ring smaller than a lookup cell");
+    if (ovf & 1u)
+        return fail(BPX_E_BUDGET,
+                    "a wavefront saved more than the %llu records its share of the scratch holds; use bpx_scan",
+                    (unsigned long long)seg_cap);
+    *num_records = h_meta[0];
+    if (stats) {
+        const unsigned long long* u = h_meta + kMetaWords;
+        stats->lookup_hits = (int64_t)u[0];
+        stats->num_seqs_lookup_hits = (int64_t)u[1];
+        stats->init_extends = (int64_t)u[2];
+        stats->good_init_extends = (int64_t)u[3];
+        stats->num_seqs_passed = (int64_t)u[4];
+    }
+    if (h_meta[0] > record_capacity)
+        return fail(BPX_E_ARG, "record capacity %llu < %llu records", (unsigned long long)record_capacity,
+                    (unsigned long long)h_meta[0]);
+    return BPX_OK;
 }
 
 int bpx_result_wait(bpx_result* r) {

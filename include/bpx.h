@@ -222,6 +222,34 @@ int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
 int bpx_result_replay(bpx_result *r);
 int bpx_result_destroy(bpx_result *r);
 
+/* ---- scan without allocation ---------------------------------------------
+ * No counterpart in the reference (which allocates its device buffers on
+ * every search, gpu_blastp.cu:139-175).  For callers that cannot allocate
+ * memory while they scan: tight serving loops, harnesses that forbid
+ * hipMalloc.  The scan of bpx_scan for one prepared query over the whole
+ * database, synchronous, with every buffer beyond its inputs and outputs in
+ * the library's fixed scratch array: one per GPU the library runs on
+ * (BPX_NOALLOC_SCRATCH_BYTES, held until the process ends).  Limits: at most
+ * BPX_NOALLOC_MAX_SUBJECTS subjects; BPX_NOALLOC_SEG_RECORDS record slots,
+ * split evenly among the launch's wavefronts -- a wavefront that saves more
+ * records than its share fails the call with BPX_E_BUDGET (bpx_scan sizes
+ * its buffers instead).  Outputs: d_records / d_subjects are device buffers
+ * of record_capacity entries (bpx_result_records' records and subject ids,
+ * in the same order); d_stats (nullable) a device buffer of one
+ * bpx_subject_stats per subject; *num_records and *stats (nullable) on the
+ * host.  When the records exceed record_capacity, none past it are written,
+ * *num_records still receives their number and the call returns BPX_E_ARG.
+ * No hipMalloc, no pinned or pageable host buffer, no allocating library
+ * call.  The scratch keeps the last call's contents (every call clears what
+ * must start at zero); calls that overlap on one GPU -- from other threads
+ * or other contexts of that device -- share it and race. */
+#define BPX_NOALLOC_MAX_SUBJECTS 2097152u
+#define BPX_NOALLOC_SEG_RECORDS 1048576u
+#define BPX_NOALLOC_SCRATCH_BYTES 58729984u
+int bpx_scan_noalloc(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, bpx_init_hsp *d_records,
+                     uint32_t *d_subjects, uint64_t record_capacity, bpx_subject_stats *d_stats,
+                     uint64_t *num_records, bpx_ungapped_stats *stats);
+
 /* ---- autotuner ----------------------------------------------------------
  * Times batched scans of subjects [sample_first, sample_first+sample_count)
  * under candidate packing policies and launch shapes and returns the
