@@ -1,34 +1,35 @@
 """Benchmark of the gfx950 blastp word finder (two-hit word finding +
-ungapped X-drop extension) on one MI355X.
+ungapped X-drop extension) on one MI355X.  Prints ONE JSON line; detail goes
+to stderr.  Inputs are resident in HBM before any timed region (database
+arena and query tables uploaded once).
 
---workload plain (default): one step = one pass of the hot path over one
-batch: the 500-residue query scanned against the resident 2,000,000-subject
-database (~720 M residues, lognormal lengths median 300 sigma 0.6, 0.5 %
-planted homologs at 30-70 % identity, 0-3 % indels), including the record
-compaction and the copy of records and per-subject counters back to the host.
+--workload headline (default): the batched, cost-ordered scan of 1,024
+queries (log-uniform 50-2,000 aa) against the 500,000-subject database
+(bpx.workloads 3).  One step = one replay of the HIP graph that holds the
+whole batched scan (counter resets, every launch class, compaction, copies)
+plus the copy of its records to the host.  `value` is DB residues/s summed
+over the 1,024 queries.  Beside it, in `config`:
+  * the plain form -- each query scanned on its own (bpx_scan), the way the
+    reference launches one query at a time -- and batched_over_plain;
+  * the same batch under the static longest-first order, and the learned
+    cost order's gain over it;
+  * the plain 500-aa query against the 2,000,000-subject database
+    (bpx.workloads 2).
+`roofline` is that of the batched step's k_scan launches (every launch
+class), timed by HIP events around each graph replay on its stream (the
+replay is k_scan but for a zero-fill, the compaction and small copies,
+under 0.1 % of it by the rocprofv3 trace).
 
---workload batched: 1,024 queries (log-uniform 50-2,000 aa) against a
-500,000-subject database (1 % homologs of random queries at 40-60 %
-identity); one step = one batched scan of all queries (autotuned packing and
-launch shape), compared with scanning the queries one by one.
-
---workload skewed: the same 1,024 queries against a skewed 500,000-subject
-database (lognormal lengths median 350 sigma 1.2, clipped 20-35,000; 20 %
-compositionally biased with tandem repeats); the learned cost model
-(bpx.costsched) orders and bins the work; one step = one batched scan,
-compared with the static longest-first order.
-
---workload serving: single queries (lognormal lengths, median 350, sigma
-0.6, clipped 20-3,000) arriving as a Poisson stream against the resident
-2,000,000-subject database, served by the continuous-batching server
-(bpx.Server); reports throughput and p50/p99 latency per arrival rate, and
-checks every result against the plain single-query scan.
-
-Inputs are resident in HBM before the timed region (database arena and query
-tables uploaded once).  Prints ONE JSON line; extra
-detail goes to stderr.
+--workload plain: the 500-aa query against the 2,000,000-subject database
+alone, one step = bpx_scan + bpx_result_wait.
+--workload skewed: the batch against the skewed 500,000-subject database
+(bpx.workloads 4), learned cost order vs longest-first.
+--workload serving: Poisson single queries against the resident
+2,000,000-subject database through the continuous-batching server; p50/p99
+latency per arrival rate, every result checked against the plain scan.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -41,84 +42,372 @@ sys.path.insert(0, os.path.join(ROOT, "blastp-cuda_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "DB residues/s on 1 GPU, batched+binned vs plain; HBM GB/s; p50/p99 latency"
+DATA = "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def plain_workload(threads):
-    import bpx
-    q = bpx.synth_query(0, 0, 500)
-    spec = dict(seed=2, num_subjects=2_000_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
-                homolog_fraction=0.005, identity_min=0.3, identity_max=0.7, indel_min=0.0, indel_max=0.03)
-    return q, bpx.synth_db(spec, [q], nthreads=threads)
+def host_cores():
+    """Threads the CPU baseline may use: every CPU this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
-def batched_workload(threads):
-    import bpx
-    lens = bpx.synth_query_lengths(3, 1024, "loguniform", 50, 2000)
-    qs = [bpx.synth_query(3, i, int(n)) for i, n in enumerate(lens)]
-    spec = dict(seed=3, num_subjects=500_000, len_median=300, len_sigma=0.6, len_min=30, len_max=5000,
-                homolog_fraction=0.01, identity_min=0.4, identity_max=0.6, indel_min=0.0, indel_max=0.03)
-    return qs, bpx.synth_db(spec, qs, nthreads=threads)
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(q, db, threads, budget_s=20.0):
+def kernel_source_sha():
+    """SHA-256 of the scan kernel's source: a measured traffic figure is used
+    only while the kernel it was measured on is unchanged."""
+    with open(os.path.join(ROOT, "blastp-cuda_amd", "csrc", "kernels.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def measured_traffic(name):
+    """HBM bytes per step from a committed rocprofv3 --pmc summary
+    (profiles/<name>.json), or (None, reason)."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return None, "not measured"
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernels_hip_sha256") != kernel_source_sha():
+        return None, f"profiles/{name}.json was measured on another kernel source"
+    return t.get("bytes_per_launch"), f"profiles/{name}.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
+
+
+def median_rate(fn, work, runs):
+    """work / median wall time of `runs` calls of fn (after one warm-up)."""
+    fn()
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    return work / float(np.median(times)), times
+
+
+# ---- CPU baselines (the C oracle; test infrastructure used as the checker's timing) ----
+
+def cpu_baseline_plain(q, db, threads, budget_s=20.0):
     """The C oracle (same semantics, same inputs) on bounded prefixes of the
-    database, timed on this host's cores: on `threads` threads, the median of
-    5 runs after a warm-up; on one thread, the median of 3 runs on a smaller
-    prefix after a warm-up."""
+    database: on `threads` threads the median of 5 runs after a warm-up, on
+    one thread the median of 3."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     lk = O.OracleLookup(q)
 
     def timed(nthreads, share, runs):
-        # size the prefix from a short probe so this leg takes ~share of budget_s
         probe = 2000
         t = time.perf_counter()
         O.run_db(q, db.residues, db.offsets[: probe + 1], db.lengths[:probe], nthreads=nthreads, lookup=lk)
         dt = max(time.perf_counter() - t, 1e-4)
-        rate = db.offsets[probe] / dt
-        target = int(rate * budget_s * share / (runs + 1))
+        target = int(db.offsets[probe] / dt * budget_s * share / (runs + 1))
         n = int(np.searchsorted(db.offsets, min(target, int(db.offsets[-1]))))
         n = max(probe, min(n, db.num_subjects))
-        times = []
-        for _ in range(runs + 1):
-            t = time.perf_counter()
-            O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=nthreads, lookup=lk)
-            times.append(time.perf_counter() - t)
         res = int(db.offsets[n])
-        return n, res, res / float(np.median(times[1:]))
+        rate, _ = median_rate(lambda: O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n],
+                                               nthreads=nthreads, lookup=lk), res, runs)
+        return n, res, rate
 
     n, res, value = timed(threads, 0.6, 5)
     n1, res1, value1 = timed(1, 0.4, 3)
     lk.close()
     return {"value": value, "unit": "DB residues/s", "cores": threads, "kind": "port",
+            "host_cpus": host_cores(), "cpu_model": host_cpu_model(),
             "sample": f"first {n} subjects ({res} residues) of the same database and query; "
                       f"median of 5 runs after a warm-up, {threads} threads",
             "value_1thread": value1,
             "sample_1thread": f"first {n1} subjects ({res1} residues); median of 3 runs after a warm-up, 1 thread"}
 
 
+def cpu_baseline_batched(qs, db, threads, budget_s=24.0):
+    """The C oracle over every 16th query of the batch against a bounded
+    prefix of the database (DB residues summed over the sampled queries per
+    second): on `threads` threads the median of 5 runs after a warm-up, on
+    one thread the median of 3 on a smaller prefix."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    sample_q = qs[::16]
+    lks = [O.OracleLookup(q) for q in sample_q]
+
+    def run(n, nthreads):
+        for q, lk in zip(sample_q, lks):
+            O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=nthreads, lookup=lk)
+
+    def timed(nthreads, share, runs):
+        # the oracle's threads claim 4,096-subject chunks (NCBI's OID chunk
+        # iterator): a sample of fewer chunks than threads would leave
+        # threads idle, so it holds at least one chunk per thread
+        probe = 300
+        t = time.perf_counter()
+        run(probe, nthreads)
+        dt = max(time.perf_counter() - t, 1e-4)
+        n = int(max(4096 * nthreads, probe * budget_s * share / (runs + 1) / dt))
+        n = min(db.num_subjects, n)
+        work = int(db.offsets[n]) * len(sample_q)
+        rate, _ = median_rate(lambda: run(n, nthreads), work, runs)
+        return n, rate
+
+    n, value = timed(threads, 0.65, 5)
+    n1, value1 = timed(1, 0.35, 3)
+    for lk in lks:
+        lk.close()
+    return {"value": value, "unit": "DB residues/s", "cores": threads, "kind": "port",
+            "host_cpus": host_cores(), "cpu_model": host_cpu_model(),
+            "sample": f"{len(sample_q)} of the 1,024 queries (every 16th) against the first {n} subjects; "
+                      f"median of 5 runs after a warm-up, {threads} threads",
+            "value_1thread": value1,
+            "sample_1thread": f"the same queries against the first {n1} subjects; median of 3 runs after a "
+                              "warm-up, 1 thread"}
+
+
+# ---- the plain 500-aa x 2,000,000-subject scan ----------------------------------------
+
+def plain_leg(ctx, threads, steps, warmup):
+    """Generates and loads workload 2, then times `steps` plain scans (each
+    bpx_scan + bpx_result_wait with per-subject counters)."""
+    import bpx
+    from bpx import workloads as W
+    t0 = time.perf_counter()
+    q, db = W.config2(threads)
+    if db.sha256 != W.SHA256[2]:
+        raise SystemExit(f"workload 2 database differs from the pinned one: {db.sha256}")
+    log(f"  workload 2: {db.num_subjects} subjects, {db.num_residues} residues ({time.perf_counter() - t0:.1f} s)")
+    t_load = time.perf_counter()
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    load_s = time.perf_counter() - t_load
+    dq = ctx.prepare_query(q)
+
+    def step():
+        r = bpx.scan(ctx, dq, ddb, subject_stats=True).wait()
+        t = r.timing()
+        n, st = r.summary()
+        r.close()
+        return t, n, st
+
+    for _ in range(warmup):
+        step()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t, nrec, ust = step()
+        kernel_ms.append(t["kernel_ms"])
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    kern = float(np.mean(kernel_ms))
+    # algorithmic HBM bytes of the launch: residues (1 B) + descriptors
+    # (12 B) read, records (24 B) written; the per-subject counters (12 B)
+    # go to pinned host memory
+    bytes_alg = db.num_residues + 12 * db.num_subjects + 24 * nrec
+    out = dict(q=q, db=db, ddb=ddb, dq=dq, ms=ms, kern=kern, kernel_ms=kernel_ms, nrec=nrec, ust=ust,
+               load_s=load_s, bytes_alg=bytes_alg, value=db.num_residues / (ms / 1000.0))
+    return out
+
+
+def plain(args):
+    import bpx
+    log("the 500-aa x 2,000,000-subject workload ...")
+    ctx = bpx.Context(0)
+    P = plain_leg(ctx, args.threads, args.steps, args.warmup)
+    db = P["db"]
+    traffic, tsrc = measured_traffic("traffic_k_scan_plain_500aa_2M_subjects")
+    achieved = P["bytes_alg"] / (P["kern"] / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": tsrc,
+                "kernel": "k_scan<3,true,false,true>", "kernel_ms_mean": round(P["kern"], 4),
+                "kernel_ms_p50": round(float(np.percentile(P["kernel_ms"], 50)), 4),
+                "kernel_ms_p99": round(float(np.percentile(P["kernel_ms"], 99)), 4),
+                "bytes_alg": int(P["bytes_alg"])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline_plain(P["q"], db, args.threads)
+    out = {"metric": METRIC, "value": round(P["value"], 1), "unit": "DB residues/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(P["ms"], 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32", "data": DATA,
+           "config": {"workload": "plain: 1 query x 500 aa vs 2,000,000-subject DB",
+                      "queries": 1, "query_len": 500, "subjects": db.num_subjects, "residues": db.num_residues,
+                      "db_sha256": db.sha256, "records": int(P["nrec"]), "lookup_hits": P["ust"]["lookup_hits"],
+                      "init_extends": P["ust"]["init_extends"], "db_load_s": round(P["load_s"], 4),
+                      "cold_search_residues_per_s": round(db.num_residues / (P["load_s"] + P["ms"] / 1000.0), 1)},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+# ---- the headline: batched + cost-ordered vs plain on the 1,024-query workload ----------
+
+def headline(args):
+    import bpx
+    from bpx import costsched
+    from bpx import workloads as W
+    log("workload 3: 1,024 queries (log-uniform 50-2,000 aa) x 500,000 subjects ...")
+    t0 = time.perf_counter()
+    qs, db = W.config3(args.threads)
+    if db.sha256 != W.SHA256[3]:
+        raise SystemExit(f"workload 3 database differs from the pinned one: {db.sha256}")
+    nq = len(qs)
+    qres = int(sum(len(q) for q in qs))
+    log(f"  {nq} queries ({qres} residues), {db.num_subjects} subjects, {db.num_residues} residues "
+        f"({time.perf_counter() - t0:.1f} s)")
+    ctx = bpx.Context(0)
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    work = db.num_residues * nq  # DB residues summed over queries
+
+    # the learned cost model: trained on 100 seeded databases drawn like this one
+    t = time.perf_counter()
+    sample = costsched.sample_queries(qs, 16)
+    model, counts, losses = costsched.fit(ctx, W.CONFIG3_SPEC, sample, batches=100, subjects=2000, epochs=8)
+    train_s = time.perf_counter() - t
+    log(f"  cost model trained in {train_s:.1f} s; epoch losses {[round(float(x), 4) for x in losses]}")
+
+    # packing policy and launch shape: autotuned on a 10 % sample
+    t = time.perf_counter()
+    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
+        f"{len(trials)} trials)")
+    for tr in trials:
+        log(f"    {tr}")
+    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    groups = b.info()["groups"]
+    t = time.perf_counter()
+    X = bpx.db_features(ctx, ddb, counts)
+    cost = costsched.predicted_cost(model, X, db.lengths, nq, len(sample), groups)
+    infer_s = time.perf_counter() - t
+    model.close()
+
+    def graph_steps(cost_order, steps, warmup):
+        bpx.db_set_cost(ddb, cost_order)
+        g = bpx.scan_batch(ctx, b, ddb, launch=launch, graph=True).wait()
+        for _ in range(warmup):
+            g.replay().wait()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.replay().wait()
+            kms.append(g.timing()["kernel_ms"])
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        n = g.summary()[0]
+        g.close()
+        return ms, kms, n
+
+    # the static longest-first order (the reference's length ordering), then
+    # the learned order: the headline, timed over exactly `steps` replays
+    lf_steps = min(args.steps, 5)
+    ms_lf, _, n_lf = graph_steps(None, lf_steps, 1)
+    log(f"  longest-first order: {ms_lf:.1f} ms/step -> {work / ms_lf / 1e6:.2f} G res/s")
+    ms, kernel_ms, nrec = graph_steps(cost, args.steps, args.warmup)
+    value = work / (ms / 1000.0)
+    log(f"  learned cost order: {ms:.1f} ms/step -> {value / 1e9:.2f} G res/s")
+    if n_lf != nrec:
+        raise SystemExit(f"record counts differ between orders: {n_lf} vs {nrec}")
+
+    # plain: every query on its own, as the reference launches one query at a
+    # time (one warm-up pass over 32 queries, then the median of 3 passes)
+    bpx.db_set_cost(ddb, None)
+    dqs = [ctx.prepare_query(q) for q in qs]
+    plain_records = [0]
+
+    def plain_pass(sub=None):
+        n = 0
+        for dq in (dqs if sub is None else dqs[:sub]):
+            r = bpx.scan(ctx, dq, ddb, subject_stats=False).wait()
+            n += r.summary()[0]
+            r.close()
+        plain_records[0] = n
+
+    plain_pass(32)
+    plain_value, plain_times = median_rate(plain_pass, work, 3)
+    if plain_records[0] != nrec:
+        raise SystemExit(f"batched and plain record counts differ: {nrec} vs {plain_records[0]}")
+    for dq in dqs:
+        dq.close()
+    log(f"  plain (each query alone): {np.median(plain_times):.2f} s -> {plain_value / 1e9:.2f} G res/s; "
+        f"batched over plain {value / plain_value:.3f}")
+    b.close()
+    ddb.close()
+
+    # the plain 500-aa query against the 2,000,000-subject database
+    P = plain_leg(ctx, args.threads, 20, 3)
+    log(f"  workload 2 plain: {P['ms']:.2f} ms/step -> {P['value'] / 1e9:.2f} G res/s (k_scan {P['kern']:.3f} ms)")
+    plain2 = {"workload": "plain: 1 query x 500 aa vs 2,000,000-subject DB", "value": round(P["value"], 1),
+              "ms_per_step": round(P["ms"], 4), "steps": 20, "records": int(P["nrec"]),
+              "db_sha256": P["db"].sha256, "kernel_ms_mean": round(P["kern"], 4),
+              "roofline_frac": round(P["bytes_alg"] / (P["kern"] / 1000.0) / 1e9 / HBM_PEAK_GBS, 5),
+              "db_load_s": round(P["load_s"], 4),
+              "cold_search_residues_per_s": round(P["db"].num_residues / (P["load_s"] + P["ms"] / 1000.0), 1)}
+    P["dq"].close()
+    P["ddb"].close()
+
+    # roofline of the batched step's k_scan launches (every launch class):
+    # each group pass reads the residues and the 12-byte descriptors, and the
+    # records (24 B) are written
+    kern = float(np.mean(kernel_ms))
+    bytes_alg = groups * (db.num_residues + 12 * db.num_subjects) + 24 * nrec
+    achieved = bytes_alg / (kern / 1000.0) / 1e9
+    traffic, tsrc = measured_traffic("traffic_k_scan_batched_1024_queries_500k_subjects")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
+                "kernel": "k_scan (all launch classes of one batched step)",
+                "kernel_ms_mean": round(kern, 3),
+                "kernel_ms_p50": round(float(np.percentile(kernel_ms, 50)), 3),
+                "kernel_ms_p99": round(float(np.percentile(kernel_ms, 99)), 3),
+                "bytes_alg": int(bytes_alg), "passes": groups}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline_batched(qs, db, args.threads)
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "DB residues/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32", "data": DATA,
+           "config": {"workload": "batched + learned cost order: 1,024 queries (log-uniform 50-2,000 aa) vs "
+                                  "500,000-subject DB",
+                      "queries": nq, "query_residues": qres, "subjects": db.num_subjects,
+                      "residues": db.num_residues, "db_sha256": db.sha256, "records": int(nrec),
+                      "groups": groups, "group_residues": gr, "group_queries": gq, "launch": launch,
+                      "plain_value": round(plain_value, 1),
+                      "plain_seconds": round(float(np.median(plain_times)), 3),
+                      "batched_over_plain": round(value / plain_value, 3),
+                      "ms_longest_first": round(ms_lf, 3), "steps_longest_first": lf_steps,
+                      "gain_over_longest_first": round(ms_lf / ms, 4),
+                      "cost_model_train_s": round(train_s, 2), "cost_model_inference_s": round(infer_s, 3),
+                      "plain_2M": plain2},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; batched 5)")
-    ap.add_argument("--warmup", type=int, default=None, help="warm-up steps (default 3; batched 1)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: headline 10, plain 20, skewed 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="warm-up steps (default: headline 1, plain 3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--workload", choices=("plain", "batched", "skewed", "serving"), default="plain")
+    ap.add_argument("--threads", type=int, default=host_cores(),
+                    help="host threads of the CPU baseline and the generator (default: every CPU available)")
+    ap.add_argument("--workload", choices=("headline", "batched", "plain", "skewed", "serving"), default="headline")
     ap.add_argument("--rates", default="25,50,200,2000", help="serving: arrival rates (queries/s), comma-separated")
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per arrival rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a plain elementwise copy (torch.neg over int32) reading and writing BYTES, "
                          "3 times: the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.workload == "batched":
+        args.workload = "headline"
     if args.steps is None:
-        args.steps = {"batched": 5, "skewed": 2}.get(args.workload, 20)
+        args.steps = {"headline": 10, "skewed": 2}.get(args.workload, 20)
     if args.warmup is None:
-        args.warmup = 1 if args.workload in ("batched", "skewed") else 3
+        args.warmup = 1 if args.workload in ("headline", "skewed") else 3
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder shards by "
               "subject across GPUs, but this build measures the single-GPU path.", file=sys.stderr)
@@ -135,206 +424,16 @@ def main():
         log(f"copy probe: {3} x torch.neg over {n} int32 ({4 * n} B read, {4 * n} B written per launch)")
         del a, b
 
-    if args.workload == "batched":
-        return batched(args)
-    if args.workload == "skewed":
-        return skewed(args)
-    if args.workload == "serving":
-        return serving(args)
-
-    import bpx
-    log("generating the 500-aa x 2,000,000-subject workload ...")
-    t0 = time.perf_counter()
-    q, db = plain_workload(args.threads)
-    log(f"  {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256} "
-        f"({time.perf_counter() - t0:.1f} s)")
-    ctx = bpx.Context(0)
-    dinfo = ctx.info()
-    t_load = time.perf_counter()
-    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
-    load_s = time.perf_counter() - t_load
-    dq = ctx.prepare_query(q)
-    log(f"  device {dinfo}; query table {dq.info()}; database load (host arena + upload) {load_s:.3f} s")
-
-    def step():
-        r = bpx.scan(ctx, dq, ddb, subject_stats=True).wait()
-        t = r.timing()
-        n, st = r.summary()
-        r.close()
-        return t, n, st
-
-    for _ in range(args.warmup):
-        step()
-    kernel_ms, device_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t, nrec, ust = step()
-        kernel_ms.append(t["kernel_ms"])
-        device_ms.append(t["device_ms"])
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
-    value = db.num_residues / (ms_per_step / 1000.0)
-
-    # roofline of the word-finder kernel: algorithmic HBM bytes per launch =
-    # residues (1 B) + subject descriptors (12 B) read + records (24 B)
-    # written; the per-subject counters (12 B) go to pinned host memory
-    kern = float(np.mean(kernel_ms))
-    bytes_alg = db.num_residues + 12 * db.num_subjects + 24 * nrec
-    achieved = bytes_alg / (kern / 1000.0) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_scan<3,true,false,true>", "kernel_ms_mean": round(kern, 4),
-                "kernel_ms_p50": round(float(np.percentile(kernel_ms, 50)), 4),
-                "kernel_ms_p99": round(float(np.percentile(kernel_ms, 99)), 4),
-                "bytes_alg": int(bytes_alg)}
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_k_scan_plain_500aa_2M_subjects.json")
-    if os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            roofline["traffic"] = json.load(f).get("bytes_per_launch")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("timing the CPU oracle on a bounded sample ...")
-        cpu = cpu_baseline(q, db, args.threads)
-
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "DB residues/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
-           "data": "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)",
-           "config": {"workload": "plain: 1 query x 500 aa vs 2,000,000-subject DB",
-                      "queries": 1, "query_len": 500, "subjects": db.num_subjects,
-                      "residues": db.num_residues, "db_sha256": db.sha256,
-                      "records": int(nrec), "lookup_hits": ust["lookup_hits"],
-                      "init_extends": ust["init_extends"], "device_ms_mean": round(float(np.mean(device_ms)), 4),
-                      # a search that first loads the volume from host memory (not `value`)
-                      "db_load_s": round(load_s, 4),
-                      "cold_search_residues_per_s": round(db.num_residues / (load_s + ms_per_step / 1000.0), 1)},
-           "roofline": roofline, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
-
-
-def cpu_baseline_batched(qs, db, threads, budget_s=20.0):
-    """The C oracle over every 16th query of the batch against a bounded
-    prefix of the database, scaled to query-residue pairs per second."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
-    sample_q = qs[::16]
-    probe = 500
-    t = time.perf_counter()
-    for q in sample_q:
-        O.run_db(q, db.residues, db.offsets[: probe + 1], db.lengths[:probe], nthreads=threads)
-    dt = max(time.perf_counter() - t, 1e-4)
-    n = int(min(db.num_subjects, max(probe, probe * budget_s / 2 / dt)))
-    t = time.perf_counter()
-    for q in sample_q:
-        O.run_db(q, db.residues, db.offsets[: n + 1], db.lengths[:n], nthreads=threads)
-    dt = time.perf_counter() - t
-    res = int(db.offsets[n]) * len(sample_q)
-    return {"value": res / dt, "unit": "DB residues/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample_q)} of the 1,024 queries (every 16th) against the first {n} subjects; "
-                      f"one timed run after a probe, {threads} threads"}
-
-
-def batched(args):
-    import bpx
-    log("generating the 1,024-query batched workload ...")
-    t0 = time.perf_counter()
-    qs, db = batched_workload(args.threads)
-    qres = int(sum(len(q) for q in qs))
-    log(f"  {len(qs)} queries ({qres} residues), {db.num_subjects} subjects, {db.num_residues} residues, "
-        f"sha256 {db.sha256} ({time.perf_counter() - t0:.1f} s)")
-    ctx = bpx.Context(0)
-    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
-    work = db.num_residues * len(qs)  # DB residues summed over queries
-
-    # autotune on a 10 % sample (subjects [0, N/10))
-    t = time.perf_counter()
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
-    log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
-        f"{len(trials)} trials)")
-    for tr in trials:
-        log(f"    {tr}")
-    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
-    info = b.info()
-    # one HIP graph holds the whole batched scan (counter resets, every launch
-    # class, compaction, result copies); each step replays it
-    graph = bpx.scan_batch(ctx, b, ddb, launch=launch, graph=True).wait()
-
-    def step():
-        graph.replay().wait()
-        return graph.timing(), graph.summary()[0]
-
-    for _ in range(args.warmup):
-        step()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tm, nrec = step()
-        kernel_ms.append(tm["kernel_ms"])
-    ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
-    value = work / (ms_per_step / 1000.0)
-    graph.close()
-
-    # plain: every query on its own (one warm-up pass over 32 queries)
-    dqs = [ctx.prepare_query(q) for q in qs]
-    for dq in dqs[:32]:
-        bpx.scan(ctx, dq, ddb, subject_stats=False).wait().close()
-    t0 = time.perf_counter()
-    plain_records = 0
-    for dq in dqs:
-        r = bpx.scan(ctx, dq, ddb, subject_stats=False).wait()
-        plain_records += r.summary()[0]
-        r.close()
-    plain_s = time.perf_counter() - t0
-    plain_value = work / plain_s
-    if plain_records != nrec:
-        raise SystemExit(f"batched and plain record counts differ: {nrec} vs {plain_records}")
-    log(f"  batched {ms_per_step:.1f} ms/step -> {value / 1e9:.2f} G res/s; plain {plain_s:.2f} s -> "
-        f"{plain_value / 1e9:.2f} G res/s; ratio {value / plain_value:.2f}")
-
-    kern = float(np.mean(kernel_ms))
-    passes = info["groups"]
-    # every group pass reads the residues and descriptors; each (query,
-    # subject) pair's record count is zero-filled, summed and turned into its
-    # CSR offset in place (12 B); records are 24 B
-    bytes_alg = passes * (db.num_residues + 12 * db.num_subjects) + 12 * db.num_subjects * len(qs) + 24 * nrec
-    achieved = bytes_alg / (kern / 1000.0) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_scan (the graph of the batched scan: all launch classes + compaction)",
-                "kernel_ms_mean": round(kern, 3),
-                "bytes_alg": int(bytes_alg), "passes": passes}
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("timing the CPU oracle on a bounded sample ...")
-        cpu = cpu_baseline_batched(qs, db, args.threads)
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "DB residues/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8/int32",
-           "data": "synthetic (seeded generator, BLOSUM62 background residues, planted homologs)",
-           "config": {"workload": "batched: 1,024 queries (log-uniform 50-2,000 aa) vs 500,000-subject DB",
-                      "queries": len(qs), "query_residues": qres, "subjects": db.num_subjects,
-                      "residues": db.num_residues, "db_sha256": db.sha256, "records": int(nrec),
-                      "groups": passes, "group_residues": gr, "group_queries": gq, "launch": launch,
-                      "plain_value": round(plain_value, 1), "plain_seconds": round(plain_s, 3),
-                      "batched_over_plain": round(value / plain_value, 3)},
-           "roofline": roofline, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
-
-
-SKEWED_SPEC = dict(seed=4, num_subjects=500_000, len_median=350, len_sigma=1.2, len_min=20, len_max=35000,
-                   homolog_fraction=0.01, identity_min=0.4, identity_max=0.6, indel_min=0.0, indel_max=0.03,
-                   biased_fraction=0.2, dirichlet_alpha=0.3, motif_min=5, motif_max=30)
+    return {"headline": headline, "plain": plain, "skewed": skewed, "serving": serving}[args.workload](args)
 
 
 def skewed(args):
     import bpx
     from bpx import costsched
+    from bpx import workloads as W
     log("generating the skewed-database workload ...")
     t0 = time.perf_counter()
-    lens = bpx.synth_query_lengths(3, 1024, "loguniform", 50, 2000)
-    qs = [bpx.synth_query(3, i, int(n)) for i, n in enumerate(lens)]
-    db = bpx.synth_db(SKEWED_SPEC, qs, nthreads=args.threads)
+    qs, db = W.config4(args.threads)
     log(f"  {len(qs)} queries, {db.num_subjects} subjects, {db.num_residues} residues, sha256 {db.sha256} "
         f"({time.perf_counter() - t0:.1f} s)")
     ctx = bpx.Context(0)
@@ -344,7 +443,7 @@ def skewed(args):
     # the cost model: trained on 100 seeded databases drawn like this one
     t = time.perf_counter()
     sample = costsched.sample_queries(qs, 16)
-    model, counts, losses = costsched.fit(ctx, SKEWED_SPEC, sample, batches=100, subjects=2000, epochs=8)
+    model, counts, losses = costsched.fit(ctx, W.CONFIG4_SPEC, sample, batches=100, subjects=2000, epochs=8)
     train_s = time.perf_counter() - t
     log(f"  cost model: trained in {train_s:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}")
 
@@ -402,14 +501,14 @@ def skewed(args):
 def serving(args):
     import threading
     import bpx
+    from bpx import workloads as W
     log("generating the 2,000,000-subject database and the request stream ...")
-    q0, db = plain_workload(args.threads)
+    q0, db = W.config2(args.threads)
     ctx = bpx.Context(0)
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
     rates = [float(x) for x in args.rates.split(",")]
     n = args.requests
-    lens = bpx.synth_query_lengths(5, n * len(rates), "lognormal", 20, 3000, 350, 0.6)
-    queries = [bpx.synth_query(5, i, int(L)) for i, L in enumerate(lens)]
+    queries = W.serving_queries(n * len(rates))
     # warm-up: one request through a server (code objects, pools)
     srv = bpx.Server(ctx, ddb)
     t = srv.submit(queries[0])
