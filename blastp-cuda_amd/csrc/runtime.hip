@@ -388,7 +388,10 @@ struct Dims {
 // tables often fit 13-15 waves (measured 2 % faster on the 1,024-query batch
 // than rounding down to 12)
 constexpr int kAutoWaves[] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 6, 4, 2, 1};
-constexpr int kLdsMinWaves = 8;
+// A table goes to LDS only when that keeps at least this many waves (or as
+// many as an HBM table): plain scans of 800-1,000-aa queries took 4.8-7.8 ms
+// with LDS tables at 8-11 waves against 4.2-4.6 ms with HBM tables at 16.
+constexpr int kLdsMinWaves = 12;
 
 // Groups are binned (one launch per bin) by coarse wave counts {16, 12, 8,
 // 4, 2, 1}; a bin then runs at the most waves in its range that all its
