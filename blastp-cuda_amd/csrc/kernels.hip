@@ -773,8 +773,24 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 if (fits && cnt) {
                     const uint32_t at = head + count + (incl - cnt);
                     const uint32_t sv = (uint32_t)(s - qbase);
-                    for (uint32_t t = 0; t < cnt; ++t)
-                        ring[(at + t) & qmask] = ((uint32_t)entries[L.start + t] << 16) | sv;
+                    if (TAB_LDS) {
+                        for (uint32_t t = 0; t < cnt; ++t)
+                            ring[(at + t) & qmask] = ((uint32_t)entries[L.start + t] << 16) | sv;
+                    } else {
+                        // a table in HBM: eight independent loads per step, so a
+                        // cell costs one memory round trip per eight entries
+                        // instead of one per entry.  Loads past the cell stay
+                        // inside the table buffer (the next group's table, or
+                        // the zero words after the last one).
+                        for (uint32_t t = 0; t < cnt; t += 8) {
+                            uint32_t v[8];
+#pragma unroll
+                            for (uint32_t k = 0; k < 8; ++k) v[k] = entries[L.start + t + k];
+#pragma unroll
+                            for (uint32_t k = 0; k < 8; ++k)
+                                if (t + k < cnt) ring[(at + t + k) & qmask] = (v[k] << 16) | sv;
+                        }
+                    }
                 }
                 // nfit is wave-uniform: a lane read, not an LDS permute
                 const uint32_t added = (uint32_t)__builtin_amdgcn_readlane((int)incl, (int)nfit - 1);
