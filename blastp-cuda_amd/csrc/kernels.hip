@@ -414,15 +414,25 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t pass = 0;               // flat: items in order, then done
     uint32_t c_next = 0, c_end = 0;  // flat: this wave's claimed subjects (wave-uniform)
     for (;;) {
-        __syncthreads();
-        if (tid == 0) {
-            ctl[0] = p.flat ? min(pass, p.nitems) : atomicAdd(p.work_counter, 1u);
-            ctl[1] = 0;
+        // Flat: every wavefront walks the items in order on its own and the
+        // workgroup meets only where the group, and with it the LDS staging,
+        // changes -- consecutive items of one group (cost bins of its
+        // subjects) run without a barrier.  Otherwise the workgroup claims
+        // items one at a time from the launch's counter.
+        uint32_t item;
+        if (p.flat) {
+            item = min(pass, p.nitems);
+        } else {
+            __syncthreads();
+            if (tid == 0) {
+                ctl[0] = atomicAdd(p.work_counter, 1u);
+                ctl[1] = 0;
+            }
+            __syncthreads();
+            item = ctl[0];
         }
         ++pass;
         c_next = c_end = 0;
-        __syncthreads();
-        const uint32_t item = ctl[0];
         if (item >= p.nitems) {
             // largest record count any wave produced: sizes a rescan after
             // a segment overflow exactly
@@ -430,7 +440,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             break;
         }
         const WorkItem w = p.items[item];
+        // subjects a wavefront claims at a time from this item (flat)
+        const uint32_t chunk = w.chunk ? w.chunk : p.flat;
         if (w.group != cur_group) {
+            // every wave is done with the previous group's LDS data
+            if (p.flat) __syncthreads();
             g = p.groups[w.group];
             if (TAB_LDS)
                 for (uint32_t i = tid; i < g.table_words; i += blockDim.x) tab_lds[i] = p.tables[g.table_word + i];
@@ -472,9 +486,9 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // every subject goes to exactly one wave, in processing order
                 if (c_next == c_end) {
                     uint32_t c = 0;
-                    if (lane == 0) c = atomicAdd(&p.flat_counters[item], p.flat);
+                    if (lane == 0) c = atomicAdd(&p.flat_counters[item], chunk);
                     c_next = (uint32_t)__builtin_amdgcn_readlane((int)c, 0);
-                    c_end = c_next + p.flat;
+                    c_end = c_next + chunk;
                 }
                 k = c_next++;
             } else {

@@ -43,7 +43,7 @@ struct WorkItem {
     uint32_t group;
     uint32_t first;   // index into the order list
     uint32_t count;
-    uint32_t pad;
+    uint32_t chunk;   // flat launches: subjects a wavefront claims at a time (0: the launch's)
 };
 
 // Record as staged in a wavefront's output segment: the HSP plus the index

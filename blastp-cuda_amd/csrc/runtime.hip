@@ -150,7 +150,13 @@ struct bpx_db {
     std::vector<uint32_t> h_len;
     std::vector<uint32_t> h_order;
     bool cost_order = false;  // h_order set by bpx_db_set_cost
+    // cost bins (set with a cost order): bin b is processing positions
+    // [bin_end[b-1], bin_end[b]), of mean predicted cost bin_weight[b] times
+    // the database's mean
+    std::vector<uint32_t> bin_end;
+    std::vector<float> bin_weight;
 };
+constexpr int kCostBins = 8;  // bins of equal predicted work per cost order
 constexpr uint32_t kCostLongPrefix = 15360;  // cost orders keep subjects this long first, longest first
 constexpr int32_t kCostMaxWindow = bpx::kNarrowLimit - kCostLongPrefix;
 
@@ -510,6 +516,8 @@ struct Planner {
 // >= kNarrowLimit) go to wide-slot launches, the rest to narrow-slot ones.
 int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide, const bpx_launch* user) {
     const Planner P(r->ctx, set, count, nwide, user);
+    // cost bins apply to scans of the whole database in its own order
+    const bpx_db* cb = (r->db && !r->db->bin_end.empty() && r->first == 0 && count == r->db->n) ? r->db : nullptr;
     struct Acc {
         std::vector<uint32_t> groups;
         Dims d;
@@ -567,7 +575,23 @@ int plan_scan(bpx_result* r, const QuerySet& set, uint32_t count, uint32_t nwide
             bpx_result::Class c = P.make_class(a.d, tab_lds, std::get<1>(ob.first) != 0, narrow, waves, lo, hi);
             r->lds_need = std::max(r->lds_need, c.plan.total);
             c.item_first = (uint32_t)r->items.size();
-            if (c.flat) {
+            if (c.flat && cb) {
+                // a scan of the whole database with cost bins: one item per
+                // (group, bin), the claim chunk scaled by the bin's mean cost
+                for (uint32_t gi : a.groups) {
+                    uint32_t f = lo;
+                    for (size_t b = 0; b < cb->bin_end.size() && f < hi; ++b) {
+                        const uint32_t e = std::min(hi, cb->bin_end[b]);
+                        if (e <= f) continue;
+                        const float w = std::max(cb->bin_weight[b], 1e-6f);
+                        const uint32_t chunk = (uint32_t)std::max<long>(
+                            1, std::min<long>(kFlatChunk, std::lround((float)c.flat / w)));
+                        r->items.push_back({gi, f, e - f, chunk});
+                        f = e;
+                    }
+                    if (f < hi) r->items.push_back({gi, f, hi - f, 0});
+                }
+            } else if (c.flat) {
                 for (uint32_t gi : a.groups) r->items.push_back({gi, lo, hi - lo, 0});
             } else {
                 for (uint32_t gi : a.groups)
@@ -1363,9 +1387,38 @@ int bpx_db_set_cost(bpx_db* db, const float* cost) {
     } else {
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return db->h_len[a] > db->h_len[b]; });
     }
+    // Bins of equal predicted work along the order: a scan of the whole
+    // database gives each its own work item, whose wavefronts claim subjects
+    // in chunks sized by the bin's mean cost (the heaviest one at a time),
+    // so a claim holds about the same work in every bin.
+    std::vector<uint32_t> bin_end;
+    std::vector<float> bin_weight;
+    if (cost && db->n) {
+        double total = 0;
+        for (uint32_t i = 0; i < db->n; ++i) total += cost[i];
+        if (total > 0) {
+            const double mean = total / db->n;
+            double acc = 0, bin_sum = 0;
+            uint32_t bin_first = 0;
+            for (uint32_t k = 0; k < db->n; ++k) {
+                const double c = cost[order[k]];
+                acc += c;
+                bin_sum += c;
+                const bool cut = acc >= total * (double)(bin_end.size() + 1) / kCostBins || k + 1 == db->n;
+                if (cut) {
+                    bin_end.push_back(k + 1);
+                    bin_weight.push_back((float)(bin_sum / (k + 1 - bin_first) / mean));
+                    bin_first = k + 1;
+                    bin_sum = 0;
+                }
+            }
+        }
+    }
     db->h_order.swap(order);
     HIP_TRY(upload_order(db));
     db->cost_order = cost != nullptr;
+    db->bin_end.swap(bin_end);
+    db->bin_weight.swap(bin_weight);
     return BPX_OK;
 }
 
