@@ -52,7 +52,7 @@ class WordParams(C.Structure):
 class Launch(C.Structure):
     _fields_ = [("waves_per_group", C.c_int32), ("groups_per_cu", C.c_int32),
                 ("subjects_per_item", C.c_int32), ("queue_capacity", C.c_int32),
-                ("subject_buffer", C.c_int32), ("reserved", C.c_int32)]
+                ("subject_buffer", C.c_int32), ("claims_per_share", C.c_int32)]
 
 
 class BatchPolicy(C.Structure):
@@ -61,7 +61,7 @@ class BatchPolicy(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_int32), ("max_inflight", C.c_int32), ("max_wait_us", C.c_int32),
-                ("reserved", C.c_int32), ("hbm_budget", C.c_int64), ("params", WordParams),
+                ("diag_budget", C.c_int32), ("hbm_budget", C.c_int64), ("params", WordParams),
                 ("policy", BatchPolicy), ("launch", Launch)]
 
 
@@ -121,6 +121,9 @@ _sig("bpx_query_prepare", C.c_int, _vp, _vp, C.c_int32, _vp, C.POINTER(WordParam
 _sig("bpx_query_info", C.c_int, _vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
      C.POINTER(C.c_int64))
 _sig("bpx_query_destroy", C.c_int, _vp)
+_sig("bpx_context_limit", C.c_int, _vp, C.c_int64)
+_sig("bpx_query_footprint", C.c_int, _vp, _vp, C.c_int32, C.POINTER(WordParams), C.POINTER(Launch),
+     C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_size_t))
 _sig("bpx_check_memory", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch),
      C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("bpx_scan", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch), C.c_uint32,
@@ -317,6 +320,10 @@ class Context:
         cus, lds, hbm = C.c_int32(), C.c_int64(), C.c_int64()
         _check(_lib.bpx_context_info(self._h, C.byref(cus), C.byref(lds), C.byref(hbm)))
         return {"num_cus": cus.value, "lds_bytes": lds.value, "hbm_bytes": hbm.value}
+
+    def limit(self, hbm_bytes: int):
+        """bpx_context_limit: cap the device bytes scans are planned with (0: the device's)."""
+        _check(_lib.bpx_context_limit(self._h, hbm_bytes))
 
     def close(self):
         if self._h:
@@ -549,6 +556,16 @@ def last_error() -> str:
     return _lib.bpx_last_error().decode(errors="replace")
 
 
+def query_footprint(ctx: Context, db: Database, qlen: int, params: WordParams | None = None, launch=None):
+    """bpx_query_footprint: (status, diagonal slots, waves, LDS bytes) of a
+    qlen-residue query scanned alone against db."""
+    d, w, lds = C.c_uint32(), C.c_int32(), C.c_size_t()
+    lp = _launch(launch)
+    rc = _lib.bpx_query_footprint(ctx._h, db._h, qlen, C.byref(params) if params is not None else None,
+                                  C.byref(lp) if lp is not None else None, C.byref(d), C.byref(w), C.byref(lds))
+    return rc, int(d.value), int(w.value), int(lds.value)
+
+
 def check_memory(ctx: Context, query: Query, db: Database, first=0, count=None, launch=None):
     if count is None:
         count = db.num_subjects - first
@@ -681,9 +698,10 @@ class Server:
     fetches its records.  Each result equals scanning the query alone."""
 
     def __init__(self, ctx: Context, db: Database, max_batch=0, max_inflight=0, max_wait_us=0, hbm_budget=0,
-                 group_residues=0, group_queries=0, launch=None, params: WordParams | None = None):
+                 group_residues=0, group_queries=0, launch=None, params: WordParams | None = None, diag_budget=0):
         cfg = ServerConfig()
         cfg.max_batch, cfg.max_inflight, cfg.max_wait_us, cfg.hbm_budget = max_batch, max_inflight, max_wait_us, hbm_budget
+        cfg.diag_budget = diag_budget
         if params is not None:
             cfg.params = params
         cfg.policy = BatchPolicy(group_residues, group_queries)

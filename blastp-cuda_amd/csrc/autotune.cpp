@@ -7,12 +7,15 @@
 // Search (coordinate descent, each step keeps the best so far):
 //   1. packing policy (group_residues, group_queries) -- how many queries
 //      share one pass over a subject, hence the per-wave diagonal-table
-//      size and the lookup-table placement;
-//   2. subject buffer bytes (per-wave LDS window over the subject).
-// The work distribution is left to the library default (wavefronts claim
-// subjects device-wide, in chunks sized for the scan at hand), which a
-// sample could not rank anyway.  Results do not depend on any of these, so
-// the tuner only reads timings.
+//      size (the sum of the members' diagonal lengths) and the lookup-table
+//      placement;
+//   2. subject buffer bytes (per-wave LDS window over the subject);
+//   3. hit-ring capacity (hits queued per wave for the diagonal table);
+//   4. subjects per wavefront claim, as claims per wavefront share: timed on
+//      four times the sample (or the whole database), because a claim's
+//      size follows the number of subjects and a small sample would give
+//      every candidate the same one-subject claims.
+// Results do not depend on any of these, so the tuner only reads timings.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -66,11 +69,12 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         }
         ++ntried;
     };
+    uint32_t first = sample_first, count = sample_count;
     auto run = [&](const Candidate& c, float* ms) -> int {
         bpx_batch* b = nullptr;
         int r = bpx_batch_prepare(ctx, num_queries, queries, qlens, matrix, params, x_dropoff, cutoff, &c.policy,
                                   &b);
-        if (r == BPX_OK) r = time_candidate(ctx, b, db, sample_first, sample_count, c.launch, ms);
+        if (r == BPX_OK) r = time_candidate(ctx, b, db, first, count, c.launch, ms);
         bpx_batch_destroy(b);
         if (r == BPX_OK) record(c, *ms);
         return r;
@@ -102,6 +106,24 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
     for (int32_t sb : {1024, 4096}) {
         Candidate c = base;
         c.launch.subject_buffer = sb;
+        if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    const Candidate base2 = best;
+    for (int32_t qc : {128, 512}) {
+        Candidate c = base2;
+        c.launch.queue_capacity = qc;
+        if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    // claim sizes on a larger range: the best so far is timed again there
+    // first, then the other claim counts
+    const uint64_t wide = std::min<uint64_t>((uint64_t)sample_count * 4, nsub);
+    first = (uint32_t)std::min<uint64_t>(sample_first, nsub - wide);
+    count = (uint32_t)wide;
+    const Candidate base3 = best;
+    best_ms = -1.f;
+    for (int32_t k : {16, 4, 8, 32}) {
+        Candidate c = base3;
+        c.launch.claims_per_share = k == 16 ? base3.launch.claims_per_share : k;
         if ((rc = consider(c)) != BPX_OK) return rc;
     }
     out->policy = best.policy;

@@ -497,15 +497,16 @@ struct Planner {
         c.diag_cap = std::max<uint32_t>(a.diag, 1);
         if (L.subjects_per_item <= 0) {
             // one item per group; every workgroup works through them in order
-            // and its waves claim subjects device-wide, a few at a time: about
-            // 1/16 of a wave's share (at most kFlatChunk), so the longest
+            // and its waves claim subjects device-wide, a few at a time: by
+            // default about 1/16 of a wave's share (at most kFlatChunk), so the longest
             // subjects, which go first, cannot pile up on a few waves, and the
             // counters see few enough atomics.  256 per-query scans of 500,000
             // subjects took 0.80 s at 8 at a time, 0.84 s at 16 and 1.35 s at
             // 32; on 2,000,000 subjects 8, 16 and 32 are equal and 1 is 2.3x
             // slower (one atomic per subject).
             const uint64_t wv = (uint64_t)ctx->num_cus * L.groups_per_cu * waves;
-            c.flat = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kFlatChunk, (hi - lo) / (wv * 16)));
+            const uint64_t k = L.claims_per_share > 0 ? (uint64_t)L.claims_per_share : 16u;
+            c.flat = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kFlatChunk, (hi - lo) / (wv * k)));
         }
         return c;
     }
@@ -1592,6 +1593,53 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
     if (rc) return rc;
     if (ctx->hbm_bytes && need > (uint64_t)ctx->hbm_bytes)
         return fail(BPX_E_BUDGET, "needs %llu device bytes", (unsigned long long)need);
+    return BPX_OK;
+}
+
+int bpx_context_limit(bpx_ctx* ctx, int64_t hbm_bytes) {
+    if (!ctx || hbm_bytes < 0) return fail(BPX_E_ARG, "NULL context or negative limit");
+    HIP_TRY(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    ctx->hbm_bytes = hbm_bytes > 0 ? std::min<int64_t>(hbm_bytes, (int64_t)tot) : (int64_t)tot;
+    return BPX_OK;
+}
+
+int bpx_query_footprint(bpx_ctx* ctx, const bpx_db* db, int32_t qlen, const bpx_word_params* params,
+                        const bpx_launch* launch, uint32_t* diag_slots, int32_t* waves, size_t* lds_bytes) {
+    if (!ctx || !db) return fail(BPX_E_ARG, "NULL handle");
+    if (qlen < 0 || qlen > 65535) return fail(BPX_E_ARG, "query length %d outside [0, 65535]", qlen);
+    bpx_word_params p{3, 11, 40, 16, 41, 0};
+    if (params && params->word_length != 0) p = *params;
+    if (p.window <= 0) return fail(BPX_E_ARG, "window must be positive");
+    const int64_t D = p.diag_mask > 0 ? (int64_t)p.diag_mask + 1 : next_pow2((int64_t)qlen + p.window);
+    if (diag_slots) *diag_slots = (uint32_t)std::min<int64_t>(D, 0xFFFFFFFFll);
+    if (waves) *waves = 0;
+    if (lds_bytes) *lds_bytes = 0;
+    if (D > 65536) return fail(BPX_E_BUDGET, "a %d-residue query needs %lld diagonal slots (at most 65536)", qlen,
+                               (long long)D);
+    // the planner's view of the query alone, its table read from HBM
+    QuerySet set;
+    set.ctx = ctx;
+    set.nq = 1;
+    set.word = p.word_length;
+    set.window = p.window;
+    set.max_cell.push_back(0);
+    bpx::GroupInfo g{};
+    g.qres_len = (uint32_t)qlen;
+    g.nq = 1;
+    g.diag_slots = (uint32_t)D;
+    set.groups.push_back(g);
+    const Planner P(ctx, set, db->n, count_wide(db, 0, db->n, p.window), launch);
+    Dims d;
+    d.add(g);
+    const int w = P.best_waves(d, false, bpx::kMaxWaves);
+    if (w == 0) {
+        uint32_t need = 0;
+        return P.refuse(d, false, P.fewest(), &need);
+    }
+    if (waves) *waves = w;
+    if (lds_bytes) *lds_bytes = P.plan_of(d, false, w, count_wide(db, 0, db->n, p.window) < db->n).total;
     return BPX_OK;
 }
 

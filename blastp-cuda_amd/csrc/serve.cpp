@@ -40,6 +40,7 @@ struct Request {
     bool done = false;
     int status = BPX_OK;
     std::string err;  // bpx_last_error() of the worker when status != BPX_OK
+    uint32_t diag = 0;  // diagonal-table slots of the query (bpx_query_footprint)
     uint32_t batch_size = 0;
     std::vector<bpx_init_hsp> recs;
     std::vector<uint32_t> subs;
@@ -126,6 +127,7 @@ void bpx_server::run() {
     const uint64_t per_query = (uint64_t)nsub * 24 + (1u << 20);
     const size_t budget_queries =
         (size_t)std::max<uint64_t>(1, (uint64_t)cfg.hbm_budget / std::max<uint64_t>(per_query, 1));
+    const uint64_t diag_budget = (uint64_t)cfg.diag_budget;
     for (;;) {
         std::vector<ReqPtr> take;
         bool idle_wait = false;
@@ -135,11 +137,18 @@ void bpx_server::run() {
                 cv_work.wait(lk, [&] { return stop || !pending.empty(); });
             if (stop && pending.empty() && fl.empty()) return;
             if (fl.size() < max_inflight && !pending.empty()) {
+                // the longest prefix of the queue within max_batch, the HBM
+                // result budget and the diagonal-table budget (a query above
+                // the latter goes alone)
+                const size_t cap = std::min((size_t)cfg.max_batch, budget_queries);
+                size_t n = 0;
+                uint64_t diag = 0;
+                while (n < pending.size() && n < cap && (n == 0 || diag + pending[n]->diag <= diag_budget))
+                    diag += pending[n++]->diag;
                 const auto oldest = ms_between(pending.front()->t_submit, Clock::now()) * 1000.0;
-                const bool launch = fl.empty() || pending.size() >= (size_t)cfg.max_batch ||
-                                    oldest >= (double)cfg.max_wait_us;
+                const bool full = n < pending.size();
+                const bool launch = fl.empty() || full || oldest >= (double)cfg.max_wait_us;
                 if (launch) {
-                    const size_t n = std::min({pending.size(), (size_t)cfg.max_batch, budget_queries});
                     take.assign(pending.begin(), pending.begin() + (ptrdiff_t)n);
                     pending.erase(pending.begin(), pending.begin() + (ptrdiff_t)n);
                 } else {
@@ -214,6 +223,7 @@ int bpx_server_create(bpx_ctx* ctx, const bpx_db* db, const bpx_server_config* c
     if (c.max_inflight <= 0) c.max_inflight = 2;
     c.max_inflight = std::min(c.max_inflight, 2);
     if (c.max_wait_us <= 0) c.max_wait_us = 10000;
+    if (c.diag_budget <= 0) c.diag_budget = 1 << 20;
     if (c.params.word_length == 0) {
         c.params.word_length = 3;
         c.params.threshold = 11;
@@ -237,10 +247,22 @@ int bpx_server_create(bpx_ctx* ctx, const bpx_db* db, const bpx_server_config* c
 }
 
 int bpx_server_submit(bpx_server* s, const uint8_t* query, int32_t qlen, uint64_t* ticket) {
-    if (!s || !ticket || qlen < 0 || qlen > 65535 || (qlen > 0 && !query)) return BPX_E_ARG;
+    if (!s || !ticket || qlen < 0 || qlen > 65535 || (qlen > 0 && !query)) {
+        bpx::set_last_error("bpx_server_submit: NULL argument or query length outside [0, 65535]");
+        return BPX_E_ARG;
+    }
     for (int32_t i = 0; i < qlen; ++i)
-        if (query[i] >= BPX_ALPHABET) return BPX_E_ARG;
+        if (query[i] >= BPX_ALPHABET) {
+            bpx::set_last_error("bpx_server_submit: residue code >= 28");
+            return BPX_E_ARG;
+        }
+    // admission: a query that cannot run against the database (diagonal
+    // table or LDS) is refused here rather than failing the batch it joins
+    uint32_t diag = 0;
+    const int fc = bpx_query_footprint(s->ctx, s->db, qlen, &s->cfg.params, &s->cfg.launch, &diag, nullptr, nullptr);
+    if (fc != BPX_OK) return fc;
     auto r = std::make_shared<Request>();
+    r->diag = diag;
     r->q.assign(query, query + qlen);
     r->t_submit = Clock::now();
     {

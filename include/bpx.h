@@ -109,7 +109,9 @@ typedef struct bpx_launch {
                                    by wavefronts device-wide, no items */
     int32_t queue_capacity;     /* per-wave LDS hit ring entries (power of two) */
     int32_t subject_buffer;     /* per-wave LDS subject buffer bytes (power of two >= 1024) */
-    int32_t reserved;
+    int32_t claims_per_share;   /* device-wide claims: a wavefront claims about 1/claims_per_share
+                                   of its share of an item's subjects at a time (at most 16
+                                   subjects); 0 = 16 */
 } bpx_launch;
 
 /* How a batch packs its queries into scan groups: queries shorter than
@@ -157,10 +159,27 @@ int bpx_query_info(const bpx_query *q, int64_t *table_bytes, int32_t *lds_reside
 int bpx_query_destroy(bpx_query *q);
 
 /* Device bytes a scan of subjects [first, first+count) of db would need and
- * the LDS bytes per workgroup; returns BPX_E_BUDGET if it does not fit. */
+ * the LDS bytes per workgroup; returns BPX_E_BUDGET if it does not fit the
+ * device (or the context's limit, bpx_context_limit). */
 int bpx_check_memory(bpx_ctx *ctx, const bpx_db *db, const bpx_query *q,
                      uint32_t first, uint32_t count, const bpx_launch *launch,
                      size_t *device_bytes, size_t *lds_bytes);
+/* Caps the device bytes bpx_check_memory (and the server's default result
+ * budget) plans with, below the device's own size -- for a GPU shared with
+ * other work.  0 restores the device size.  The check of GPU_BLASTP_check_memory
+ * (gpu_blastp.cu:316-333) against totalGlobalMem, with the total settable. */
+int bpx_context_limit(bpx_ctx *ctx, int64_t hbm_bytes);
+/* Footprint of a query of qlen residues scanned alone against db, planned
+ * without building its table (the table read from HBM, the placement every
+ * query can take): its diagonal slots, the most wavefronts per workgroup it
+ * runs at and the LDS bytes per workgroup there.  BPX_E_BUDGET if it cannot
+ * run at all (more than 65,536 diagonal slots, or not even one wavefront's
+ * buffers fit the LDS) -- the shared-memory and diagonal-array checks of
+ * gpu_blastp.cu:63-65, 184-191, made before any work is queued.  params as
+ * for bpx_query_prepare (word_length 0: blastp defaults); launch nullable. */
+int bpx_query_footprint(bpx_ctx *ctx, const bpx_db *db, int32_t qlen, const bpx_word_params *params,
+                        const bpx_launch *launch, uint32_t *diag_slots, int32_t *waves,
+                        size_t *lds_bytes);
 
 /* ---- batch: many queries scanned in one persistent launch ---------------
  * queries[i] has qlens[i] residues.  params gives word_length, threshold,
@@ -255,9 +274,13 @@ int bpx_scan_noalloc(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, bpx_ini
  * under candidate packing policies and launch shapes and returns the
  * fastest.  No reference counterpart: the reference takes its launch shape
  * from -gpu_blocks / -gpu_threads (cmdline_flags.cpp:218-222).  Coordinate
- * search: policy, then subject buffer, then subjects per work item.  The
- * arguments before db are those of bpx_batch_prepare.  trials (nullable)
- * receives every configuration timed. */
+ * search, each step keeping the best so far: packing policy (which sets each
+ * group's per-wavefront LDS diagonal table, the sum of its members'
+ * diagonal lengths, and its table placement), subject buffer, hit-ring
+ * capacity, then subjects per wavefront claim (claims_per_share), the last
+ * timed on four times the sample (or the whole database) since the claim
+ * size follows the number of subjects.  The arguments before db are those of
+ * bpx_batch_prepare.  trials (nullable) receives every configuration timed. */
 typedef struct bpx_tune_trial {
     bpx_batch_policy policy;
     bpx_launch launch;
@@ -305,9 +328,11 @@ int bpx_costmodel_grad(bpx_costmodel *cm, const float *features, const float *ta
 const char *bpx_costmodel_last_error(void);
 int bpx_costmodel_destroy(bpx_costmodel *cm);
 /* Processing order by predicted cost: subjects of length >= 15,360 first
- * (longest first), then the rest by descending cost; work items stay
- * equal-count ranges of that order.  cost = NULL restores the default
- * longest-first order.  Scans with a window above 1,024 are refused while a
+ * (longest first), then the rest by descending cost.  The order is cut into
+ * 8 bins of equal predicted work; a scan of the whole database gives each
+ * bin its own work item, whose wavefronts claim subjects in chunks sized by
+ * the bin's mean cost.  cost = NULL restores the default longest-first
+ * order without bins.  Scans with a window above 1,024 are refused while a
  * cost order is set.  Not while a scan of this database is in flight. */
 int bpx_db_set_cost(bpx_db *db, const float *cost);
 
@@ -315,10 +340,12 @@ int bpx_db_set_cost(bpx_db *db, const float *cost);
  * No reference counterpart (blastp-cuda serves one search per process).
  * Continuous batching over a resident database: submitted queries wait in a
  * queue; a worker thread admits them into batched scans when a device slot
- * is free (at most max_inflight batches, on two stream lanes) and the
- * batch's result buffers fit hbm_budget -- at once when the device is idle,
- * otherwise when max_batch queries wait or the oldest has waited
- * max_wait_us.  Each request's records and stats are exactly those of
+ * is free (at most max_inflight batches, on two stream lanes), the batch's
+ * result buffers fit hbm_budget and its queries' diagonal tables fit
+ * diag_budget -- at once when the device is idle, otherwise when max_batch
+ * queries wait or the oldest has waited max_wait_us.  A query that cannot
+ * run against the database at all (bpx_query_footprint) is refused by
+ * bpx_server_submit with BPX_E_BUDGET instead of failing its batch.  Each request's records and stats are exactly those of
  * scanning it alone.  While the server runs, its worker thread owns the
  * context: make no other calls on that context from other threads. */
 typedef struct bpx_server bpx_server;
@@ -326,7 +353,8 @@ typedef struct bpx_server_config {
     int32_t max_batch;          /* queries per batch (0 = 256) */
     int32_t max_inflight;       /* batches on the device at once, 1..2 (0 = 2) */
     int32_t max_wait_us;        /* gathering time while a batch runs (0 = 10,000) */
-    int32_t reserved;
+    int32_t diag_budget;        /* diagonal-table slots (summed over its queries) one batch may
+                                   take (0 = 1,048,576); a query above it runs in a batch alone */
     int64_t hbm_budget;         /* result-buffer bytes one batch may take (0 = 1/8 of HBM) */
     bpx_word_params params;     /* word_length 0 = blastp defaults; diag_mask must be 0 */
     bpx_batch_policy policy;

@@ -322,6 +322,46 @@ def test_infeasible_launch_is_refused(ctx):
     assert rc == -5 and lds > ctx.info()["lds_bytes"]
 
 
+def test_device_memory_limit_refuses_a_scan_that_would_not_fit(ctx):
+    """GPU_BLASTP_check_memory (gpu_blastp.cu:316-333) compares the sum of a
+    search's buffers with the device size; with the context limited below
+    that sum, bpx_check_memory must return BPX_E_BUDGET, and pass again once
+    the limit is lifted."""
+    q = bpx.synth_query(4, 1, 300)
+    spec = dict(seed=8, num_subjects=2000, len_median=300, len_sigma=0.6, len_min=30, len_max=3000)
+    sdb = bpx.synth_db(spec, [q], nthreads=THREADS)
+    db = ctx.load_db(sdb.residues, sdb.offsets[:-1], sdb.lengths)
+    dq = ctx.prepare_query(q)
+    rc, need, _ = bpx.check_memory(ctx, dq, db)
+    assert rc == 0 and need > 0
+    try:
+        ctx.limit(need - 1)
+        rc, need2, _ = bpx.check_memory(ctx, dq, db)
+        assert rc == -5 and need2 == need
+        assert "device bytes" in bpx.last_error()
+    finally:
+        ctx.limit(0)
+    assert bpx.check_memory(ctx, dq, db)[0] == 0
+    dq.close()
+    db.close()
+
+
+def test_query_footprint(ctx):
+    """The footprint of a query scanned alone: its diagonal slots (next power
+    of two >= qlen + window), the waves it runs at with its table in HBM; a
+    query needing more than 65,536 slots is refused."""
+    spec = dict(seed=9, num_subjects=300, len_median=300, len_sigma=0.6, len_min=30, len_max=3000)
+    sdb = bpx.synth_db(spec)
+    db = ctx.load_db(sdb.residues, sdb.offsets[:-1], sdb.lengths)
+    rc, slots, waves, lds = bpx.query_footprint(ctx, db, 500)
+    assert rc == 0 and slots == 1024 and waves == 16 and 0 < lds <= ctx.info()["lds_bytes"]
+    rc, slots, waves, _ = bpx.query_footprint(ctx, db, 30000)
+    assert rc == 0 and slots == 32768 and 1 <= waves < 16
+    rc, slots, waves, _ = bpx.query_footprint(ctx, db, 65500)
+    assert rc == -5 and slots == 131072 and waves == 0
+    db.close()
+
+
 def test_database_load_checks_codes_and_handles_empty(ctx):
     """The arena is repacked by host threads in chunks; the first bad residue
     code (lowest subject, then position) is reported, and an empty volume

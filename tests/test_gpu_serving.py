@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import bpx
+import pyoracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -85,5 +86,39 @@ def test_bad_requests_are_refused(ctx):
         srv.submit(np.full(10, 40, dtype=np.uint8))  # residue code >= 28
     with pytest.raises(bpx.BpxError):
         srv.wait(12345, 10)  # unknown ticket
+    srv.close()
+    db.close()
+
+
+def test_diagonal_budget_splits_batches_and_results_equal_the_oracle(ctx):
+    """Admission by the diagonal-table budget: with room for four 512-slot
+    queries per batch, a burst of 24 queries is served in batches of at most
+    four, and every result equals the oracle.  A query too long to run at
+    all is refused at submit, and the others are not affected."""
+    qs = [bpx.synth_query(57, i, 300 + 6 * i) for i in range(24)]  # 300-438 aa: 512 diagonal slots each
+    spec = dict(seed=58, num_subjects=6000, len_median=300, len_sigma=0.6, len_min=30, len_max=3000,
+                homolog_fraction=0.03, identity_min=0.4, identity_max=0.7, indel_min=0.0, indel_max=0.02)
+    sdb = bpx.synth_db(spec, qs, nthreads=16)
+    db = ctx.load_db(sdb.residues, sdb.offsets[:-1], sdb.lengths)
+    srv = bpx.Server(ctx, db, diag_budget=4 * 512, max_wait_us=20000)
+    with pytest.raises(bpx.BpxError, match="diagonal slots"):
+        srv.submit(np.full(65500, 1, dtype=np.uint8))
+    tickets = [srv.submit(q) for q in qs]
+    sizes = []
+    for q, t in zip(qs, tickets):
+        info = srv.wait(t)
+        recs, subs = srv.records(t, info["num_records"])
+        sizes.append(info["batch_size"])
+        counters, recoff, orecs = O.run_db(q, sdb.residues, sdb.offsets, sdb.lengths, None, 16)
+        assert len(recs) == len(orecs)
+        np.testing.assert_array_equal(recs.view(np.int32).reshape(-1, 6), orecs.view(np.int32).reshape(-1, 6))
+        np.testing.assert_array_equal(subs, np.repeat(np.arange(sdb.num_subjects, dtype=np.uint32),
+                                                      np.diff(recoff.astype(np.int64))))
+        has = counters[:, 0] > 0
+        assert info["stats"]["lookup_hits"] == int(counters[has, 0].sum())
+        assert info["stats"]["init_extends"] == int(counters[has, 2].sum())
+    c = srv.counters()
+    say(f"batch sizes {sizes}; counters {c}")
+    assert max(sizes) <= 4 and c["batches"] >= 6 and c["completed"] == len(qs)
     srv.close()
     db.close()
