@@ -450,11 +450,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 for (uint32_t i = tid; i < g.table_words; i += blockDim.x) tab_lds[i] = p.tables[g.table_word + i];
             for (uint32_t i = tid; i < g.qres_len; i += blockDim.x) Q[i] = p.qres[g.qres_off + i];
             if (MULTI)
-                for (uint32_t i = tid; i < g.qres_len; i += blockDim.x) {
-                    uint32_t j = 0;
-                    while (j + 1 < g.nq && p.qinfo[g.qinfo_first + j + 1].qstart <= i) ++j;
-                    qmap[i] = (uint8_t)j;
-                }
+                for (uint32_t i = tid; i < g.qres_len; i += blockDim.x) qmap[i] = p.qmap[g.qres_off + i];
             for (uint32_t i = tid; i < g.nq; i += blockDim.x) qinfo[i] = p.qinfo[g.qinfo_first + i];
             cur_group = w.group;
             __syncthreads();
@@ -1044,9 +1040,45 @@ __global__ __launch_bounds__(kCompactThreads) void k_offsets(CompactParams p) {
         }
 }
 
+// Record order within a (group, subject) run: query, subject offset, query
+// offset.  Keys are unique (a second hit gives at most one record per query).
+__device__ __forceinline__ bool rec_before(const SegRecord& a, const SegRecord& b) {
+    return a.j < b.j || (a.j == b.j && (a.h.subject_offset < b.h.subject_offset ||
+                                        (a.h.subject_offset == b.h.subject_offset &&
+                                         a.h.query_offset < b.h.query_offset)));
+}
+
+// In-place heapsort of run[0, n): O(n log n) for the long runs of
+// low-complexity subjects, where an insertion sort would be quadratic (a
+// packed group's queries interleave in emission order).
+__device__ void heap_sort_run(SegRecord* run, uint32_t n) {
+    auto sift = [&](uint32_t i, uint32_t end) {
+        const SegRecord r = run[i];
+        for (;;) {
+            uint32_t c = 2 * i + 1;
+            if (c >= end) break;
+            if (c + 1 < end && rec_before(run[c], run[c + 1])) ++c;
+            if (!rec_before(r, run[c])) break;
+            run[i] = run[c];
+            i = c;
+        }
+        run[i] = r;
+    };
+    for (uint32_t i = n / 2; i-- > 0;) sift(i, n);
+    for (uint32_t end = n - 1; end > 0; --end) {
+        const SegRecord top = run[0];
+        run[0] = run[end];
+        run[end] = top;
+        sift(0, end);
+    }
+}
+
+constexpr uint32_t kInsertionMax = 64;  // longer runs are heap-sorted
+
 // One (group, subject) run of records: order it by (query, subject offset,
 // query offset) -- the sequential emission order within each query -- and
-// move each record to its CSR position.
+// move each record to its CSR position.  Runs are nearly in order already
+// for one query, so short ones take an insertion sort.
 __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p) {
     if (*p.overflow) return;  // segments incomplete; the host rescans
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1055,19 +1087,18 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
     if (len == 0) return;
     const uint32_t g = (uint32_t)(i / p.count), slot = (uint32_t)(i % p.count);
     SegRecord* run = p.seg + p.run_loc[i];
-    for (uint32_t a = 1; a < len; ++a) {
-        const SegRecord r = run[a];
-        uint32_t m = a;
-        while (m > 0) {
-            const SegRecord& o = run[m - 1];
-            const bool before = o.j < r.j || (o.j == r.j && (o.h.subject_offset < r.h.subject_offset ||
-                                                              (o.h.subject_offset == r.h.subject_offset &&
-                                                               o.h.query_offset < r.h.query_offset)));
-            if (before) break;
-            run[m] = run[m - 1];
-            --m;
+    if (len > kInsertionMax) {
+        heap_sort_run(run, len);
+    } else {
+        for (uint32_t a = 1; a < len; ++a) {
+            const SegRecord r = run[a];
+            uint32_t m = a;
+            while (m > 0 && !rec_before(run[m - 1], r)) {
+                run[m] = run[m - 1];
+                --m;
+            }
+            run[m] = r;
         }
-        run[m] = r;
     }
     const GroupInfo gi = p.groups[g];
     uint32_t cur_j = 0xFFFFFFFFu, dst = 0, qid = 0;

@@ -69,6 +69,7 @@ struct ScanParams {
     const QueryInfo* qinfo;
     const uint32_t* tables;
     const uint8_t* qres;
+    const uint8_t* qmap;        // per group position: index of its query in the group (beside qres)
     const int8_t* matrix;       // 28 x 28, row = query residue
     uint32_t ngroups_pv;        // presence-vector words (nwords / 32), same for all groups
     int32_t word, window;

@@ -174,6 +174,7 @@ struct QuerySet {
     bpx::QueryInfo* d_qinfo = nullptr;
     uint32_t* d_tables = nullptr;
     uint8_t* d_qres = nullptr;
+    uint8_t* d_qmap = nullptr;  // the query of each group position, laid out like d_qres
     int8_t* d_matrix = nullptr;
 
     void release() {
@@ -181,11 +182,13 @@ struct QuerySet {
         (void)hipFree(d_qinfo);
         (void)hipFree(d_tables);
         (void)hipFree(d_qres);
+        (void)hipFree(d_qmap);
         (void)hipFree(d_matrix);
         d_groups = nullptr;
         d_qinfo = nullptr;
         d_tables = nullptr;
         d_qres = nullptr;
+        d_qmap = nullptr;
         d_matrix = nullptr;
     }
 };
@@ -295,7 +298,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
         members.push_back(std::move(m));
     }
     std::vector<uint32_t> tables;
-    std::vector<uint8_t> qres;
+    std::vector<uint8_t> qres, qmap;
     for (const auto& m : members) {
         std::vector<bpx::QuerySpan> spans;
         bpx::GroupInfo gi{};
@@ -316,6 +319,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
             qi.qid = id;
             set.qinfo.push_back(qi);
             qres.insert(qres.end(), q.seq, q.seq + q.len);
+            qmap.insert(qmap.end(), (size_t)q.len, (uint8_t)(set.qinfo.size() - 1 - gi.qinfo_first));
             qstart += (uint32_t)q.len;
             dbase += (uint32_t)q.diag_len;
         }
@@ -339,12 +343,14 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
     // clamped rank without a branch and may touch one word past a group
     tables.insert(tables.end(), 16, 0u);
     qres.insert(qres.end(), 16, 0u);
+    qmap.insert(qmap.end(), 16, 0u);
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t ng = std::max<size_t>(set.groups.size(), 1), nqi = std::max<size_t>(set.qinfo.size(), 1);
     if (hipMalloc(&set.d_groups, ng * sizeof(bpx::GroupInfo)) != hipSuccess ||
         hipMalloc(&set.d_qinfo, nqi * sizeof(bpx::QueryInfo)) != hipSuccess ||
         hipMalloc(&set.d_tables, std::max<size_t>(tables.size(), 1) * 4) != hipSuccess ||
         hipMalloc(&set.d_qres, std::max<size_t>(qres.size(), 1)) != hipSuccess ||
+        hipMalloc(&set.d_qmap, std::max<size_t>(qmap.size(), 1)) != hipSuccess ||
         hipMalloc(&set.d_matrix, 28 * 28) != hipSuccess) {
         set.release();
         return fail(BPX_E_NOMEM, "device allocation for query tables failed");
@@ -358,6 +364,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
     if (e == hipSuccess && !tables.empty())
         e = hipMemcpy(set.d_tables, tables.data(), tables.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && !qres.empty()) e = hipMemcpy(set.d_qres, qres.data(), qres.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !qmap.empty()) e = hipMemcpy(set.d_qmap, qmap.data(), qmap.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(set.d_matrix, M, 28 * 28, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set.release();
@@ -773,6 +780,7 @@ int enqueue_job(const ScanJob& j) {
         sp.qinfo = set.d_qinfo;
         sp.tables = set.d_tables;
         sp.qres = set.d_qres;
+        sp.qmap = set.d_qmap;
         sp.matrix = set.d_matrix;
         sp.ngroups_pv = set.ngroups_pv;
         sp.word = set.word;

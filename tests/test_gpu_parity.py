@@ -308,7 +308,10 @@ def test_low_complexity_chains_and_queue_caps(ctx, launch):
         subjects.append(s)
     res, offs, lens = pack(subjects)
     say(f"low complexity, launch {launch}: {int(lens.sum())} residues")
-    assert_parity(gpu_run(ctx, q, res, offs, lens, launch=launch), oracle_run(q, res, offs, lens))
+    gpu = gpu_run(ctx, q, res, offs, lens, launch=launch)
+    assert_parity(gpu, oracle_run(q, res, offs, lens))
+    # runs of more than 64 records per subject take the compaction's heapsort
+    assert int(np.bincount(gpu[2]).max()) > 64
 
 
 def test_infeasible_launch_is_refused(ctx):
@@ -540,3 +543,32 @@ def test_volumes_load_as_one_arena(ctx, tmp_path):
     assert_parity((st, recs, subs, ustats, None), oracle_run(q, db.residues, db.offsets, db.lengths))
     dq.close()
     vdb.close()
+
+
+def test_imported_reference_gpu_volume_scans_like_the_oracle(ctx, tmp_path):
+    """A database in the reference's own .gpu / .gpuinfo files -- laid out by
+    the test suite's restatement of its writer (functions.cpp:364-613, 1-46)
+    -- is imported (bpx_gpu_volume_read), loaded and scanned on the GPU, and
+    equals the oracle on the same sequences."""
+    from test_reference_gpu_db import write_reference_files
+    q = bpx.synth_query(62, 0, 380)
+    spec = dict(seed=63, num_subjects=3000, len_median=300, len_sigma=0.6, len_min=30, len_max=4000,
+                homolog_fraction=0.03, identity_min=0.4, identity_max=0.7, indel_min=0.0, indel_max=0.02)
+    sdb = bpx.synth_db(spec, [q], nthreads=THREADS)
+    order = np.argsort(sdb.lengths, kind="stable")  # the reference's files hold sequences by ascending length
+    seqs = [sdb.residues[int(sdb.offsets[i]):int(sdb.offsets[i + 1])] for i in order]
+    path = write_reference_files(str(tmp_path), "ref", [(seqs, len(seqs))], blocks=8, threads=32)
+    res, offs, lens, _ = bpx.gpu_volume_read(path, 0)
+    assert len(lens) == len(seqs)
+    db = ctx.load_db(res, offs[:-1], lens)
+    dq = ctx.prepare_query(q)
+    r = bpx.scan(ctx, dq, db).wait()
+    recs, subs = r.records()
+    st = r.subject_stats()
+    _, ustats = r.summary()
+    r.close()
+    ores, ooffs, olens = pack(seqs)
+    assert_parity((st, recs, subs, ustats, None), oracle_run(q, ores, ooffs, olens))
+    assert len(recs) > 0
+    dq.close()
+    db.close()
