@@ -132,8 +132,9 @@ static int bpx_volume_read_impl(const char* base, bpx_volume** out) {
     // anything is sized by n (n = 2^32 - 1 would otherwise wrap n + 1)
     const uint64_t n1 = (uint64_t)v->n + 1;
     if (pin.size() < c.at || pin.size() - c.at < 8 * n1) {
+        const uint32_t n = v->n;
         delete v;
-        return vfail(BPX_E_ARG, b + ".pin: truncated offset arrays (" + std::to_string(v->n) + " sequences)");
+        return vfail(BPX_E_ARG, b + ".pin: truncated offset arrays (" + std::to_string(n) + " sequences)");
     }
     c.at += 4 * n1;
     std::vector<uint32_t> so(n1);
