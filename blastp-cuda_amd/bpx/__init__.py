@@ -33,6 +33,7 @@ STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 
 OUT_SUBJECT_STATS = 1
 OUT_GRAPH = 2
+OUT_PHASES = 4
 
 HSP_DTYPE = np.dtype([("hsp_q", "<i4"), ("hsp_s", "<i4"), ("query_offset", "<i4"),
                       ("subject_offset", "<i4"), ("hsp_len", "<i4"), ("score", "<i4")])
@@ -71,6 +72,12 @@ class TuneTrial(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("policy", BatchPolicy), ("launch", Launch), ("sample_ms", C.c_float), ("candidates", C.c_int32)]
+
+
+class PhaseStats(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("classes", C.c_uint32), ("span_ms", C.c_double),
+                ("wave_ms_mean", C.c_double), ("wave_ms_min", C.c_double), ("stage_ms_mean", C.c_double),
+                ("group_ms_mean", C.c_double), ("subjects_staged", C.c_uint64), ("bytes_staged", C.c_uint64)]
 
 
 class UngappedStats(C.Structure):
@@ -144,6 +151,7 @@ _sig("bpx_scan_batch", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER
      C.POINTER(_vp))
 _sig("bpx_result_destroy", C.c_int, _vp)
 _sig("bpx_result_replay", C.c_int, _vp)
+_sig("bpx_result_phases", C.c_int, _vp, C.POINTER(PhaseStats))
 _sig("bpx_result_ready", C.c_int, _vp)
 _sig("bpx_server_create", C.c_int, _vp, _vp, C.POINTER(ServerConfig), C.POINTER(_vp))
 _sig("bpx_server_submit", C.c_int, _vp, _vp, C.c_int32, C.POINTER(C.c_uint64))
@@ -490,6 +498,12 @@ class Result:
         _check(_lib.bpx_result_subject_stats(self._h, _ptr(st), len(st)))
         return st[:n]
 
+    def phases(self) -> dict:
+        """bpx_result_phases (scan made with phases=True)."""
+        st = PhaseStats()
+        _check(_lib.bpx_result_phases(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def replay(self) -> "Result":
         """Re-runs a graph-mode scan (same buffers); call .wait() again."""
         _check(_lib.bpx_result_replay(self._h))
@@ -508,12 +522,12 @@ class Result:
 
 
 def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | None = None,
-         launch=None, subject_stats: bool = True, graph: bool = False) -> Result:
+         launch=None, subject_stats: bool = True, graph: bool = False, phases: bool = False) -> Result:
     """Asynchronous scan of subjects [first, first+count); call .wait().
     graph=True captures the device work in a HIP graph (Result.replay)."""
     if count is None:
         count = db.num_subjects - first
-    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0)
+    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0) | (OUT_PHASES if phases else 0)
     h = _vp()
     lp = _launch(launch)
     _check(_lib.bpx_scan(ctx._h, query._h, db._h, first, count, C.byref(lp) if lp is not None else None,
@@ -522,12 +536,12 @@ def scan(ctx: Context, query: Query, db: Database, first: int = 0, count: int | 
 
 
 def scan_batch(ctx: Context, batch: Batch, db: Database, first: int = 0, count: int | None = None,
-               launch=None, subject_stats: bool = False, graph: bool = False) -> Result:
+               launch=None, subject_stats: bool = False, graph: bool = False, phases: bool = False) -> Result:
     """Asynchronous scan of every query of the batch in one persistent launch
     per launch class; graph=True captures it in a HIP graph (Result.replay)."""
     if count is None:
         count = db.num_subjects - first
-    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0)
+    flags = (OUT_SUBJECT_STATS if subject_stats else 0) | (OUT_GRAPH if graph else 0) | (OUT_PHASES if phases else 0)
     h = _vp()
     lp = _launch(launch)
     _check(_lib.bpx_scan_batch(ctx._h, batch._h, db._h, first, count, C.byref(lp) if lp is not None else None,

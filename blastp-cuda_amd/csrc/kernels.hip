@@ -407,6 +407,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     const int32_t window = p.window;
     const uint32_t seg_base = (p.seg_wave_base + blockIdx.x * nwaves + wave) * p.seg_cap;
     uint32_t seg_fill = 0;      // wave-uniform
+    // optional per-wave phase times (p.phase), on the 100 MHz real-time clock:
+    // lifetime, staging of subjects HBM -> LDS, group changes (barriers and
+    // restaging)
+    const bool timed = p.phase != nullptr;
+    const uint64_t t_begin = timed ? __builtin_amdgcn_s_memrealtime() : 0;
+    uint64_t t_stage = 0, t_group = 0, n_staged = 0, b_staged = 0;
     int32_t doff = window;      // running NCBI diagonal offset, wave-uniform
     uint32_t cur_group = 0xFFFFFFFFu;
     GroupInfo g{};
@@ -437,11 +443,23 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // largest record count any wave produced: sizes a rescan after
             // a segment overflow exactly
             if (lane == 0 && seg_fill > p.seg_cap) atomicMax(p.max_fill, seg_fill);
+            if (timed && lane == 0) {
+                unsigned long long* ph = p.phase + 8 * (size_t)(p.seg_wave_base + blockIdx.x * nwaves + wave);
+                ph[0] = t_begin;
+                ph[1] = __builtin_amdgcn_s_memrealtime();
+                ph[2] = t_stage;
+                ph[3] = t_group;
+                ph[4] = n_staged;
+                ph[5] = b_staged;
+                ph[6] = 0;
+                ph[7] = 0;
+            }
             break;
         }
         const WorkItem w = p.items[item];
         // subjects a wavefront claims at a time from this item (flat)
         const uint32_t chunk = w.chunk ? w.chunk : p.flat;
+        const uint64_t t_grp0 = timed ? __builtin_amdgcn_s_memrealtime() : 0;
         if (w.group != cur_group) {
             // every wave is done with the previous group's LDS data
             if (p.flat) __syncthreads();
@@ -454,6 +472,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             for (uint32_t i = tid; i < g.nq; i += blockDim.x) qinfo[i] = p.qinfo[g.qinfo_first + i];
             cur_group = w.group;
             __syncthreads();
+            if (timed) t_group += __builtin_amdgcn_s_memrealtime() - t_grp0;
         }
         const uint32_t* table = TAB_LDS ? tab_lds : p.tables + g.table_word;
         const uint2* pvr = reinterpret_cast<const uint2*>(table);
@@ -474,9 +493,15 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             m_nseq[t] = m_ext[t] = m_saved[t] = m_passed[t] = 0;
         }
         unsigned long long a_hits = 0;
+        // flat: the descriptors of the claimed chunk, lane j holding those of
+        // its j-th subject -- loaded with the claim, so a subject's residue
+        // loads wait for one memory round trip, not two
+        uint32_t c_first = 0, d_sid = 0, d_len = 0;
+        uint64_t d_off = 0;
 
         for (;;) {
             uint32_t k = 0;
+            const uint64_t t_st0 = timed ? __builtin_amdgcn_s_memrealtime() : 0;
             if (p.flat) {
                 // a chunk of the item's subjects from its device-wide counter:
                 // every subject goes to exactly one wave, in processing order
@@ -485,6 +510,16 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     if (lane == 0) c = atomicAdd(&p.flat_counters[item], chunk);
                     c_next = (uint32_t)__builtin_amdgcn_readlane((int)c, 0);
                     c_end = c_next + chunk;
+                    c_first = c_next;
+                    // descriptors are stored in processing order: a chunk
+                    // reads consecutive entries (chunk <= 16 lanes)
+                    const uint32_t kk = c_next + lane;
+                    if (lane < chunk && kk < w.count) {
+                        const uint32_t o = w.first + kk;
+                        d_sid = p.order[o];
+                        d_off = p.poff[o];
+                        d_len = p.plen[o];
+                    }
                 }
                 k = c_next++;
             } else {
@@ -492,13 +527,23 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 k = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
             }
             if (k >= w.count) break;
-            // descriptors are stored in processing order: consecutive
-            // subjects of an item read consecutive entries
-            const uint32_t opos = w.first + k;  // position in the processing order
-            const uint32_t sid = p.order[opos];
+            uint32_t sid;
+            uint64_t soff;
+            int32_t slen;
+            if (p.flat) {
+                const int j = (int)(k - c_first);  // wave-uniform
+                sid = (uint32_t)__builtin_amdgcn_readlane((int)d_sid, j);
+                soff = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)d_off, j) |
+                       ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(d_off >> 32), j) << 32);
+                slen = __builtin_amdgcn_readlane((int)d_len, j);
+            } else {
+                const uint32_t opos = w.first + k;  // position in the processing order
+                sid = p.order[opos];
+                soff = p.poff[opos];
+                slen = (int32_t)p.plen[opos];
+            }
             const uint32_t slot = sid - p.first;
-            const uint8_t* __restrict__ S = p.residues + p.poff[opos];
-            const int32_t slen = (int32_t)p.plen[opos];
+            const uint8_t* __restrict__ S = p.residues + soff;
             const int32_t send = (slen + 15) & ~15;
 
             // ---- stage the subject (or its first window) into LDS ----
@@ -514,6 +559,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             int32_t whi = min(send, (int32_t)p.sbuf);
             fill(0, whi);
             wave_sync();
+            if (timed) {
+                t_stage += __builtin_amdgcn_s_memrealtime() - t_st0;
+                n_staged += 1;
+                b_staged += (uint64_t)whi;
+            }
 
             uint32_t hits = 0, extended = 0, saved = 0;   // single-query counters
             const uint32_t rec_start = seg_fill;
@@ -832,10 +882,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         empty_ring = full_drain = false;
                         if (at_end) break;
                         if (pos + 128 > whi && whi < send) {
+                            const uint64_t t_w0 = timed ? __builtin_amdgcn_s_memrealtime() : 0;
                             const int32_t nhi = min(send, whi + (int32_t)(p.sbuf >> 1));
                             fill(whi, nhi);
+                            b_staged += timed ? (uint64_t)(nhi - whi) : 0u;
                             whi = nhi;
                             wave_sync();
+                            if (timed) t_stage += __builtin_amdgcn_s_memrealtime() - t_w0;
                         }
                         if (count == 0 && ndef == 0 && ntasks == 0) {
                             qbase = pos;

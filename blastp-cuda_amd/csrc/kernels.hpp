@@ -93,6 +93,8 @@ struct ScanParams {
     uint32_t seg_cap;           // records per wave segment
     uint32_t seg_wave_base;     // first wave segment of this launch
     uint32_t* overflow;         // bit 0: a segment filled; bit 1: ring too small
+    unsigned long long* phase;  // optional: 8 words per wave (launch-wide wave index, as the segments):
+                                // start, end, staging, group changes, subjects and bytes staged
     uint32_t* max_fill;         // largest per-wave record count, when above seg_cap
     // LDS layout (bytes) and shape
     uint32_t lds_table, lds_qres, lds_qmap, lds_qinfo, lds_matrix, lds_wave, wave_stride;

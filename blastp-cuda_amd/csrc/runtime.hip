@@ -227,6 +227,7 @@ struct bpx_result {
     uint32_t* d_subjects = nullptr;
     uint32_t* d_queries = nullptr;
     unsigned long long* d_meta = nullptr;  // [0] total, [1] overflow word, [8..] 5 stats per query
+    unsigned long long* d_phase = nullptr; // BPX_OUT_PHASES: 8 words per wave
     uint32_t* d_order = nullptr;           // owned only for sub-range scans
     uint64_t* d_poff = nullptr;            // (sub-range) descriptors in processing order
     uint32_t* d_plen = nullptr;
@@ -627,6 +628,8 @@ void result_release(bpx_result* r, bool keep_order) {
     pool.put(r->d_subjects);
     pool.put(r->d_queries);
     pool.put(r->d_meta);
+    pool.put(r->d_phase);
+    r->d_phase = nullptr;
     pool.put(r->d_items);
     pool.put(r->d_work);
     r->d_work = nullptr;
@@ -693,6 +696,9 @@ int alloc_buffers(bpx_result* r, uint32_t seg_cap) {
     r->d_subjects = (uint32_t*)pool.get(r->seg_total * 4);
     r->d_queries = (uint32_t*)pool.get(r->seg_total * 4);
     r->d_meta = (unsigned long long*)pool.get((kMetaWords + 5ull * set.nq) * 8);
+    if ((r->flags & BPX_OUT_PHASES) &&
+        !(r->d_phase = (unsigned long long*)pool.get(std::max<uint64_t>(total_waves(r), 1) * 64)))
+        return fail(BPX_E_NOMEM, "allocation for phase times failed");
     r->d_items = (bpx::WorkItem*)pool.get(std::max<size_t>(r->items.size(), 1) * sizeof(bpx::WorkItem));
     r->d_work = (uint32_t*)pool.get((kMaxClasses + r->items.size()) * 4);
     r->h_meta = (unsigned long long*)r->ctx->pinned.get((kMetaWords + 5ull * set.nq) * 8);
@@ -735,6 +741,7 @@ struct ScanJob {
     bpx::WorkItem* items = nullptr;  // on the device
     uint32_t* work = nullptr;
     unsigned long long* h_meta = nullptr;  // host copy of the meta words
+    unsigned long long* phase = nullptr;   // optional per-wave phase times
     hipEvent_t ev_start = nullptr, ev_kernels = nullptr;  // timing, optional
     bool capturing = false;  // the stream is being captured: kernel attributes are set beforehand
 };
@@ -802,6 +809,7 @@ int enqueue_job(const ScanJob& j) {
         sp.seg_cap = j.seg_cap;
         sp.seg_wave_base = (uint32_t)(seg_at / j.seg_cap);
         sp.overflow = overflow;
+        sp.phase = j.phase;
         sp.max_fill = reinterpret_cast<uint32_t*>(j.meta + 2);
         sp.lds_table = c.plan.table;
         sp.lds_qres = c.plan.qres;
@@ -878,6 +886,7 @@ ScanJob result_job(bpx_result* r, const uint32_t* order, bool events, bool start
     j.items = r->d_items;
     j.work = r->d_work;
     j.h_meta = r->h_meta;
+    j.phase = r->d_phase;
     if (events) {
         if (start_event) j.ev_start = r->ev[0];
         j.ev_kernels = r->ev[1];
@@ -1914,6 +1923,48 @@ int bpx_result_replay(bpx_result* r) {
     HIP_TRY(hipEventRecord(r->ev[0], r->stream));
     HIP_TRY(hipGraphLaunch(r->graph_exec, r->stream));
     HIP_TRY(hipEventRecord(r->ev[1], r->stream));
+    return BPX_OK;
+}
+
+int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
+    if (!r || !out) return fail(BPX_E_ARG, "NULL argument");
+    if (!r->waited) return fail(BPX_E_STATE, "result not waited");
+    if (!(r->flags & BPX_OUT_PHASES) || !r->d_phase) return fail(BPX_E_STATE, "scan made without BPX_OUT_PHASES");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    const uint64_t nw = total_waves(r);
+    std::vector<unsigned long long> h(nw * 8);
+    if (nw) HIP_TRY(hipMemcpy(h.data(), r->d_phase, nw * 64, hipMemcpyDeviceToHost));
+    constexpr double kTickMs = 1e-5;  // 100 MHz
+    bpx_phase_stats st{};
+    st.waves = (uint32_t)nw;
+    st.classes = (uint32_t)r->classes.size();
+    double life = 0, stage = 0, group = 0, life_min = 0;
+    uint64_t w0 = 0;
+    for (const auto& c : r->classes) {
+        const uint64_t cw = (uint64_t)r->ctx->num_cus * r->launch.groups_per_cu * c.waves;
+        uint64_t first = ~0ull, last = 0;
+        for (uint64_t w = w0; w < w0 + cw; ++w) {
+            const unsigned long long* p = &h[w * 8];
+            first = std::min<uint64_t>(first, p[0]);
+            last = std::max<uint64_t>(last, p[1]);
+            const double l = (double)(p[1] - p[0]) * kTickMs;
+            life += l;
+            life_min = w == 0 ? l : std::min(life_min, l);
+            stage += (double)p[2] * kTickMs;
+            group += (double)p[3] * kTickMs;
+            st.subjects_staged += p[4];
+            st.bytes_staged += p[5];
+        }
+        if (cw) st.span_ms += (double)(last - first) * kTickMs;
+        w0 += cw;
+    }
+    if (nw) {
+        st.wave_ms_mean = life / (double)nw;
+        st.wave_ms_min = life_min;
+        st.stage_ms_mean = stage / (double)nw;
+        st.group_ms_mean = group / (double)nw;
+    }
+    *out = st;
     return BPX_OK;
 }
 

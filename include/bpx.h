@@ -126,6 +126,7 @@ typedef struct bpx_batch_policy {
 /* Output selection for a scan. */
 #define BPX_OUT_SUBJECT_STATS 1u /* per-subject counters to the host (the kernel writes them to pinned memory) */
 #define BPX_OUT_GRAPH 2u         /* capture the scan's device work in a HIP graph (bpx_result_replay) */
+#define BPX_OUT_PHASES 4u        /* per-wavefront phase times of the scan kernels (bpx_result_phases) */
 
 /* ---- device / context ------------------------------------------------- */
 int bpx_available(void);
@@ -239,6 +240,28 @@ int bpx_result_subject_stats(const bpx_result *r, bpx_subject_stats *stats,
  * Timing then covers the whole graph.  No reference counterpart: the
  * reference re-issues every copy and launch per search (gpu_blastp.cu:139-201). */
 int bpx_result_replay(bpx_result *r);
+/* Phase times of a scan made with BPX_OUT_PHASES, from each wavefront's
+ * records on the device's 100 MHz real-time clock, summed over the scan's
+ * launch classes.  No reference counterpart (its only timer is around the
+ * launch, functions.cpp:683-695).  stage_ms_mean is the mean time a
+ * wavefront spent staging subjects from HBM into LDS (descriptor reads to
+ * residues in LDS); bytes_staged / stage_ms_mean is the rate of that
+ * streaming phase while it runs.  group_ms_mean is the mean time at group
+ * changes (waiting for the workgroup's other wavefronts, restaging the
+ * group's table and residues).  wave_ms_* are wavefront lifetimes; span_ms
+ * runs from the first start to the last end of each launch class, summed. */
+typedef struct bpx_phase_stats {
+    uint32_t waves;
+    uint32_t classes;
+    double span_ms;
+    double wave_ms_mean;
+    double wave_ms_min;
+    double stage_ms_mean;
+    double group_ms_mean;
+    uint64_t subjects_staged;
+    uint64_t bytes_staged;
+} bpx_phase_stats;
+int bpx_result_phases(const bpx_result *r, bpx_phase_stats *out);
 int bpx_result_destroy(bpx_result *r);
 
 /* ---- scan without allocation ---------------------------------------------

@@ -572,3 +572,29 @@ def test_imported_reference_gpu_volume_scans_like_the_oracle(ctx, tmp_path):
     assert len(recs) > 0
     dq.close()
     db.close()
+
+
+def test_phase_times_are_consistent_and_change_nothing(ctx):
+    """BPX_OUT_PHASES: every subject is staged once per group pass, at least
+    its residues' bytes; wavefront lifetimes lie within the launch span and
+    contain the staging and group-change times; the results are those of the
+    same scan without phase times."""
+    q, db = workload_db(1)
+    ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
+    dq = ctx.prepare_query(q)
+    plain = bpx.scan(ctx, dq, ddb).wait()
+    want = plain.records()[0].view(np.int32).copy()
+    plain.close()
+    r = bpx.scan(ctx, dq, ddb, phases=True).wait()
+    np.testing.assert_array_equal(r.records()[0].view(np.int32), want)
+    ph = r.phases()
+    say(f"phases: {ph}")
+    nw = ctx.info()["num_cus"] * ph["waves"] // ctx.info()["num_cus"]
+    assert ph["waves"] == nw and ph["waves"] % ctx.info()["num_cus"] == 0
+    assert ph["subjects_staged"] == db.num_subjects
+    assert db.num_residues <= ph["bytes_staged"] <= db.num_residues + 16 * db.num_subjects
+    assert 0 < ph["stage_ms_mean"] <= ph["wave_ms_mean"] <= ph["span_ms"] * 1.001
+    assert 0 <= ph["group_ms_mean"] <= ph["wave_ms_mean"]
+    r.close()
+    dq.close()
+    ddb.close()
