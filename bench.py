@@ -272,9 +272,10 @@ def headline(args):
     train_s = time.perf_counter() - t
     log(f"  cost model trained in {train_s:.1f} s; epoch losses {[round(float(x), 4) for x in losses]}")
 
-    # packing policy and launch shape: autotuned on a 10 % sample
+    # packing policy and launch shape: autotuned on the whole database (a
+    # group pass's fixed costs weigh more on a sample and skew the choice)
     t = time.perf_counter()
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
     log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
         f"{len(trials)} trials)")
     for tr in trials:
@@ -447,7 +448,7 @@ def skewed(args):
     train_s = time.perf_counter() - t
     log(f"  cost model: trained in {train_s:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}")
 
-    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
     b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
     groups = b.info()["groups"]
     log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch}; {groups} groups")

@@ -297,11 +297,12 @@ int bpx_scan_noalloc(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, bpx_ini
  * under candidate packing policies and launch shapes and returns the
  * fastest.  No reference counterpart: the reference takes its launch shape
  * from -gpu_blocks / -gpu_threads (cmdline_flags.cpp:218-222).  Coordinate
- * search, each step keeping the best so far: packing policy (which sets each
- * group's per-wavefront LDS diagonal table, the sum of its members'
- * diagonal lengths, and its table placement), subject buffer, hit-ring
+ * search, each step keeping the best so far: packing policy -- residues per
+ * group, then queries per group (which bound each group's per-wavefront LDS
+ * diagonal table, the sum of its members' diagonal lengths, and set its
+ * table placement) -- then subject buffer, hit-ring
  * capacity, then subjects per wavefront claim (claims_per_share), the last
- * timed on four times the sample (or the whole database) since the claim
+ * timed on twice the sample (or the whole database) since the claim
  * size follows the number of subjects.  The arguments before db are those of
  * bpx_batch_prepare.  trials (nullable) receives every configuration timed. */
 typedef struct bpx_tune_trial {
