@@ -75,9 +75,11 @@ def kernel_source_sha():
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def measured_traffic(name):
+def measured_traffic(name, policy=None):
     """HBM bytes per step from a committed rocprofv3 --pmc summary
-    (profiles/<name>.json), or (None, reason)."""
+    (profiles/<name>.json), or (None, reason).  A batched figure holds for
+    the packing policy (residues and queries per group) it was measured
+    under only."""
     path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None, "not measured"
@@ -85,6 +87,8 @@ def measured_traffic(name):
         t = json.load(f)
     if t.get("kernels_hip_sha256") != kernel_source_sha():
         return None, f"profiles/{name}.json was measured on another kernel source"
+    if policy is not None and t.get("policy") != policy:
+        return None, f"profiles/{name}.json was measured under packing policy {t.get('policy')}, not {policy}"
     return t.get("bytes_per_launch"), f"profiles/{name}.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
 
 
@@ -273,13 +277,17 @@ def headline(args):
     log(f"  cost model trained in {train_s:.1f} s; epoch losses {[round(float(x), 4) for x in losses]}")
 
     # packing policy and launch shape: autotuned on the whole database (a
-    # group pass's fixed costs weigh more on a sample and skew the choice)
+    # group pass's fixed costs weigh more on a sample and skew the choice),
+    # or given with --policy
     t = time.perf_counter()
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
-    log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
-        f"{len(trials)} trials)")
-    for tr in trials:
-        log(f"    {tr}")
+    if args.policy:
+        gr, gq, qc, sb, cps = (int(x) for x in args.policy.split(","))
+        launch = {"queue_capacity": qc, "subject_buffer": sb, "claims_per_share": cps}
+        log(f"  policy given: group_residues={gr} group_queries={gq} launch={launch}")
+    else:
+        gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
+        log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
+            f"{len(trials)} trials, each printed above as it finished)")
     b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
     groups = b.info()["groups"]
     t = time.perf_counter()
@@ -290,6 +298,17 @@ def headline(args):
 
     def graph_steps(cost_order, steps, warmup):
         bpx.db_set_cost(ddb, cost_order)
+        if args.eager:
+            kms, n = [], 0
+            t0 = time.perf_counter()
+            for i in range(warmup + steps):
+                if i == warmup:
+                    kms, t0 = [], time.perf_counter()
+                r = bpx.scan_batch(ctx, b, ddb, launch=launch).wait()
+                kms.append(r.timing()["kernel_ms"])
+                n = r.summary()[0]
+                r.close()
+            return (time.perf_counter() - t0) * 1000.0 / steps, kms, n
         g = bpx.scan_batch(ctx, b, ddb, launch=launch, graph=True).wait()
         for _ in range(warmup):
             g.replay().wait()
@@ -306,14 +325,22 @@ def headline(args):
     # the static longest-first order (the reference's length ordering), then
     # the learned order: the headline, timed over exactly `steps` replays
     lf_steps = min(args.steps, 5)
-    ms_lf, _, n_lf = graph_steps(None, lf_steps, 1)
-    log(f"  longest-first order: {ms_lf:.1f} ms/step -> {work / ms_lf / 1e6:.2f} G res/s")
+    ms_lf = None
+    if not args.only_batched:
+        ms_lf, _, n_lf = graph_steps(None, lf_steps, 1)
+        log(f"  longest-first order: {ms_lf:.1f} ms/step -> {work / ms_lf / 1e6:.2f} G res/s")
     ms, kernel_ms, nrec = graph_steps(cost, args.steps, args.warmup)
     value = work / (ms / 1000.0)
-    log(f"  learned cost order: {ms:.1f} ms/step -> {value / 1e9:.2f} G res/s")
-    if n_lf != nrec:
+    log(f"  learned cost order: {ms:.1f} ms/step -> {value / 1e9:.2f} G res/s (graph device time "
+        f"{np.mean(kernel_ms):.1f} ms)")
+    if ms_lf is not None and n_lf != nrec:
         raise SystemExit(f"record counts differ between orders: {n_lf} vs {nrec}")
 
+    if args.only_batched:
+        print(json.dumps({"workload": "batched only (profiling run)", "ms_per_step": round(ms, 3),
+                          "value": round(value, 1), "kernel_ms_mean": round(float(np.mean(kernel_ms)), 3),
+                          "records": int(nrec), "groups": groups}), flush=True)
+        return
     # plain: every query on its own, as the reference launches one query at a
     # time (one warm-up pass over 32 queries, then the median of 3 passes)
     bpx.db_set_cost(ddb, None)
@@ -357,7 +384,8 @@ def headline(args):
     kern = float(np.mean(kernel_ms))
     bytes_alg = groups * (db.num_residues + 12 * db.num_subjects) + 24 * nrec
     achieved = bytes_alg / (kern / 1000.0) / 1e9
-    traffic, tsrc = measured_traffic("traffic_k_scan_batched_1024_queries_500k_subjects")
+    traffic, tsrc = measured_traffic("traffic_k_scan_batched_1024_queries_500k_subjects",
+                                   {"group_residues": gr, "group_queries": gq})
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": "k_scan (all launch classes of one batched step)",
@@ -399,6 +427,15 @@ def main():
     ap.add_argument("--workload", choices=("headline", "batched", "plain", "skewed", "serving"), default="headline")
     ap.add_argument("--rates", default="25,50,200,2000", help="serving: arrival rates (queries/s), comma-separated")
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per arrival rate")
+    ap.add_argument("--policy", default="", metavar="GR,GQ,QCAP,SBUF,CLAIMS",
+                    help="headline: group residues, group queries, hit-ring capacity, subject buffer and claims "
+                         "per share to use instead of autotuning")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="headline: time the batched graph only (no longest-first, per-query or 2,000,000-subject "
+                         "legs), for profiling runs; prints a reduced line")
+    ap.add_argument("--eager", action="store_true",
+                    help="with --only-batched: launch each step's kernels directly instead of replaying a graph "
+                         "(the same kernels, for counter-collection runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a plain elementwise copy (torch.neg over int32) reading and writing BYTES, "
                          "3 times: the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -409,6 +446,7 @@ def main():
         args.steps = {"headline": 10, "skewed": 2}.get(args.workload, 20)
     if args.warmup is None:
         args.warmup = 1 if args.workload in ("headline", "skewed") else 3
+    os.environ.setdefault("BPX_TUNE_LOG", "1")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder shards by "
               "subject across GPUs, but this build measures the single-GPU path.", file=sys.stderr)

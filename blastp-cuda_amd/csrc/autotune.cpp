@@ -23,6 +23,8 @@
 //      every candidate the same one-subject claims.
 // Results do not depend on any of these, so the tuner only reads timings.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -79,6 +81,9 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         ++ntried;
     };
     uint32_t first = sample_first, count = sample_count;
+    // BPX_TUNE_LOG set and not "0": one line on stderr per timed candidate
+    const char* env = std::getenv("BPX_TUNE_LOG");
+    const bool log = env && *env && std::strcmp(env, "0") != 0;
     auto run = [&](const Candidate& c, float* ms) -> int {
         bpx_batch* b = nullptr;
         int r = bpx_batch_prepare(ctx, num_queries, queries, qlens, matrix, params, x_dropoff, cutoff, &c.policy,
@@ -86,6 +91,11 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         if (r == BPX_OK) r = time_candidate(ctx, b, db, first, count, c.launch, ms);
         bpx_batch_destroy(b);
         if (r == BPX_OK) record(c, *ms);
+        if (log)
+            std::fprintf(stderr, "bpx autotune: residues %u queries %u ring %u buffer %u claims %u: %s %.2f ms\n",
+                         c.policy.group_residues, c.policy.group_queries, c.launch.queue_capacity,
+                         c.launch.subject_buffer, c.launch.claims_per_share, r == BPX_OK ? "ok" : "refused",
+                         r == BPX_OK ? *ms : 0.f);
         return r;
     };
 

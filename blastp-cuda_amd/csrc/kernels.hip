@@ -409,10 +409,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t seg_fill = 0;      // wave-uniform
     // optional per-wave phase times (p.phase), on the 100 MHz real-time clock:
     // lifetime, staging of subjects HBM -> LDS, group changes (barriers and
-    // restaging)
+    // restaging), extension flushes (with the number of extensions run)
     const bool timed = p.phase != nullptr;
     const uint64_t t_begin = timed ? __builtin_amdgcn_s_memrealtime() : 0;
-    uint64_t t_stage = 0, t_group = 0, n_staged = 0, b_staged = 0;
+    uint64_t t_stage = 0, t_group = 0, n_staged = 0, b_staged = 0, t_flush = 0, n_flushed = 0;
     int32_t doff = window;      // running NCBI diagonal offset, wave-uniform
     uint32_t cur_group = 0xFFFFFFFFu;
     GroupInfo g{};
@@ -451,8 +451,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 ph[3] = t_group;
                 ph[4] = n_staged;
                 ph[5] = b_staged;
-                ph[6] = 0;
-                ph[7] = 0;
+                ph[6] = t_flush;
+                ph[7] = n_flushed;
             }
             break;
         }
@@ -692,6 +692,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // (64 / kSeg per pass); more run one per lane.
             auto flush = [&]() {
                 const uint32_t nt = ntasks;
+                const uint64_t t_f0 = timed ? __builtin_amdgcn_s_memrealtime() : 0;
                 wave_sync();
                 constexpr uint32_t kPer = 64u / kSeg;
                 const bool coop = resident && nt <= kCoopMax;
@@ -761,6 +762,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 }
                 ntasks = 0;
                 wave_sync();
+                if (timed) {
+                    t_flush += __builtin_amdgcn_s_memrealtime() - t_f0;
+                    n_flushed += nt;
+                }
             };
 
             // Positions are consumed 64 at a time into the ring.  Each pass

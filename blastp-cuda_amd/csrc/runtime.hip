@@ -1938,7 +1938,7 @@ int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
     bpx_phase_stats st{};
     st.waves = (uint32_t)nw;
     st.classes = (uint32_t)r->classes.size();
-    double life = 0, stage = 0, group = 0, life_min = 0;
+    double life = 0, stage = 0, group = 0, flush = 0, life_min = 0;
     uint64_t w0 = 0;
     for (const auto& c : r->classes) {
         const uint64_t cw = (uint64_t)r->ctx->num_cus * r->launch.groups_per_cu * c.waves;
@@ -1954,6 +1954,8 @@ int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
             group += (double)p[3] * kTickMs;
             st.subjects_staged += p[4];
             st.bytes_staged += p[5];
+            flush += (double)p[6] * kTickMs;
+            st.extensions += p[7];
         }
         if (cw) st.span_ms += (double)(last - first) * kTickMs;
         w0 += cw;
@@ -1963,6 +1965,7 @@ int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
         st.wave_ms_min = life_min;
         st.stage_ms_mean = stage / (double)nw;
         st.group_ms_mean = group / (double)nw;
+        st.flush_ms_mean = flush / (double)nw;
     }
     *out = st;
     return BPX_OK;

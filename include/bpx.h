@@ -248,8 +248,11 @@ int bpx_result_replay(bpx_result *r);
  * residues in LDS); bytes_staged / stage_ms_mean is the rate of that
  * streaming phase while it runs.  group_ms_mean is the mean time at group
  * changes (waiting for the workgroup's other wavefronts, restaging the
- * group's table and residues).  wave_ms_* are wavefront lifetimes; span_ms
- * runs from the first start to the last end of each launch class, summed. */
+ * group's table and residues).  flush_ms_mean is the mean time in
+ * extension flushes (the queued two-hit X-drop extensions and the writing
+ * of their records); extensions counts the extensions those flushes ran.
+ * wave_ms_* are wavefront lifetimes; span_ms runs from the first start to
+ * the last end of each launch class, summed. */
 typedef struct bpx_phase_stats {
     uint32_t waves;
     uint32_t classes;
@@ -260,6 +263,8 @@ typedef struct bpx_phase_stats {
     double group_ms_mean;
     uint64_t subjects_staged;
     uint64_t bytes_staged;
+    double flush_ms_mean;
+    uint64_t extensions;
 } bpx_phase_stats;
 int bpx_result_phases(const bpx_result *r, bpx_phase_stats *out);
 int bpx_result_destroy(bpx_result *r);
@@ -304,7 +309,9 @@ int bpx_scan_noalloc(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, bpx_ini
  * capacity, then subjects per wavefront claim (claims_per_share), the last
  * timed on twice the sample (or the whole database) since the claim
  * size follows the number of subjects.  The arguments before db are those of
- * bpx_batch_prepare.  trials (nullable) receives every configuration timed. */
+ * bpx_batch_prepare.  trials (nullable) receives every configuration timed.
+ * With the environment variable BPX_TUNE_LOG set (and not "0") each timed
+ * candidate also prints one line on stderr as it finishes. */
 typedef struct bpx_tune_trial {
     bpx_batch_policy policy;
     bpx_launch launch;

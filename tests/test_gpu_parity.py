@@ -595,6 +595,9 @@ def test_phase_times_are_consistent_and_change_nothing(ctx):
     assert db.num_residues <= ph["bytes_staged"] <= db.num_residues + 16 * db.num_subjects
     assert 0 < ph["stage_ms_mean"] <= ph["wave_ms_mean"] <= ph["span_ms"] * 1.001
     assert 0 <= ph["group_ms_mean"] <= ph["wave_ms_mean"]
+    # every extension the scan ran went through a flush
+    assert ph["extensions"] == r.summary()[1]["init_extends"] > 0
+    assert 0 < ph["flush_ms_mean"] < ph["wave_ms_mean"]
     r.close()
     dq.close()
     ddb.close()
