@@ -67,6 +67,7 @@ constexpr int32_t kOffsetLimit = 1 << 28;
 constexpr uint32_t kSeg = 8;       // lanes per cooperative extension task
 constexpr uint32_t kCoopMax = 16;  // flushes of at most this many tasks (of a resident subject) run cooperatively
 constexpr int kMatrixStride = 32;        // LDS score rows padded 28 -> 32
+constexpr int kMatrixRows = 32;          // ... and 28 -> 32 rows: any 10-bit index is inside
 constexpr uint32_t kCtlBytes = 16;       // workgroup control words at LDS offset 0
 
 __device__ __forceinline__ void wave_sync() {
@@ -225,6 +226,116 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
     return r;
 }
 
+// extend_two_hit for a subject resident in sbuf, software-pipelined: each
+// iteration applies two steps whose scores were loaded one iteration
+// earlier, while the next two steps' scores and the residues of the two
+// after them are in flight, so a lane waits on no load of its own
+// iteration.  Loads run up to five steps past a walk's end: subject reads
+// wrap inside sbuf, query reads stay within six bytes of the group's
+// residues (inside the LDS allocation); their values are never used.  (A
+// general form with two, three or four steps per iteration measured slower.)
+template <int WORD>
+__device__ __forceinline__ TwoHit extend_resident(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
+                                                  const uint8_t* sbuf, uint32_t smask, int32_t slen, int32_t s_left,
+                                                  int32_t s, int32_t c, int32_t xdrop) {
+    int32_t sc = 0, best = 0, rd = 0;
+#pragma unroll
+    for (int32_t i = 0; i < WORD; ++i) {
+        sc += M[Q[c + i] * kMatrixStride + sbuf[(uint32_t)(s + i) & smask]];
+        if (sc > best) {
+            best = sc;
+            rd = i + 1;
+        }
+    }
+    const int32_t cr = c + rd, sr = s + rd, qr = cr - qstart;
+    // residues past a walk's end may hold any byte: the index stays inside
+    // the 32 x 32 LDS score table (a residue code is below 28)
+    auto score_at = [&](uint32_t qb, uint32_t sb) { return (int32_t)M[((qb << 5) | sb) & 1023u]; };
+
+    // left: step k at (cr-1-k, sr-1-k), k = 0..n
+    const int32_t n = min(sr - 1, qr - 1);
+    int32_t mx = 0, left_len = 0;
+    if (n >= 0) {
+        const uint8_t* qp = Q + (cr - 1);
+        const uint32_t sp = (uint32_t)(sr - 1);
+        int32_t m0 = score_at(qp[0], sbuf[sp & smask]);
+        int32_t m1 = score_at(qp[-1], sbuf[(sp - 1u) & smask]);
+        uint32_t q2 = qp[-2], q3 = qp[-3], s2 = sbuf[(sp - 2u) & smask], s3 = sbuf[(sp - 3u) & smask];
+        int32_t score = 0;
+        bool done = false;
+        for (int32_t k0 = 0; !done; k0 += 2) {
+            const int32_t n0 = score_at(q2, s2), n1 = score_at(q3, s3);
+            q2 = qp[-(k0 + 4)];
+            q3 = qp[-(k0 + 5)];
+            s2 = sbuf[(sp - (uint32_t)(k0 + 4)) & smask];
+            s3 = sbuf[(sp - (uint32_t)(k0 + 5)) & smask];
+            // step k0 (always inside the walk), then k0 + 1 unless it ended
+            const int32_t a = score + m0;
+            const bool up0 = a > mx;
+            mx = up0 ? a : mx;
+            left_len = up0 ? k0 + 1 : left_len;
+            const bool end0 = k0 >= n || mx - a >= xdrop;
+            const int32_t b = a + m1;
+            const bool up1 = !end0 && b > mx;
+            mx = up1 ? b : mx;
+            left_len = up1 ? k0 + 2 : left_len;
+            done = end0 || k0 + 1 >= n || mx - b >= xdrop;
+            score = b;
+            m0 = n0;
+            m1 = n1;
+        }
+    }
+    TwoHit r;
+    r.right_extend = left_len >= sr - s_left;
+    r.s_last = sr;
+    int32_t right_len = 0, total = mx;
+    if (r.right_extend) {
+        // right: step k at (cr+k, sr+k), k = 0..n2-1, from the left score
+        const int32_t n2 = min(slen - sr, qlen - qr);
+        int32_t rmax = mx, best_k = -1, kstop = n2;
+        if (n2 > 0) {
+            const uint8_t* qp = Q + cr;
+            const uint32_t sp = (uint32_t)sr;
+            int32_t m0 = score_at(qp[0], sbuf[sp & smask]);
+            int32_t m1 = score_at(qp[1], sbuf[(sp + 1u) & smask]);
+            uint32_t q2 = qp[2], q3 = qp[3], s2 = sbuf[(sp + 2u) & smask], s3 = sbuf[(sp + 3u) & smask];
+            int32_t rscore = mx;
+            bool done = false;
+            for (int32_t k0 = 0; !done; k0 += 2) {
+                const int32_t n0 = score_at(q2, s2), n1 = score_at(q3, s3);
+                q2 = qp[k0 + 4];
+                q3 = qp[k0 + 5];
+                s2 = sbuf[(sp + (uint32_t)(k0 + 4)) & smask];
+                s3 = sbuf[(sp + (uint32_t)(k0 + 5)) & smask];
+                const int32_t a = rscore + m0;
+                const bool up0 = a > rmax;
+                rmax = up0 ? a : rmax;
+                best_k = up0 ? k0 : best_k;
+                const bool stop0 = a <= 0 || rmax - a >= xdrop;
+                const bool end0 = stop0 || k0 + 1 >= n2;
+                const int32_t b = a + m1;
+                const bool up1 = !end0 && b > rmax;
+                rmax = up1 ? b : rmax;
+                best_k = up1 ? k0 + 1 : best_k;
+                const bool stop1 = !end0 && (b <= 0 || rmax - b >= xdrop);
+                kstop = stop0 ? k0 : (stop1 ? k0 + 1 : kstop);
+                done = end0 || stop1 || k0 + 2 >= n2;
+                rscore = b;
+                m0 = n0;
+                m1 = n1;
+            }
+        }
+        right_len = best_k + 1;
+        r.s_last = sr + kstop;
+        total = rmax;
+    }
+    r.hsp_q = qr - left_len;
+    r.hsp_s = sr - left_len;
+    r.len = left_len + right_len;
+    r.score = total;
+    return r;
+}
+
 // Inclusive prefix sum / maximum inside each W-lane segment (W = 8 or 16) of
 // the wave: DPP row_shr 1, 2, 4 (and 8 for W = 16); a lane whose source lies
 // in the previous segment (or outside the 16-lane row) takes the identity.
@@ -278,15 +389,24 @@ __device__ __forceinline__ void seg_xdrop(const int8_t* M, const uint8_t* Q, uin
     const uint32_t rl = lane & (W - 1u), rbase = lane & ~(W - 1u);
     int32_t carry = start, mx = start, best = -1, stop_at = nsteps;
     bool done = nsteps <= 0;
+    // left steps walk down from (cq, cs), right steps walk up from it.  The
+    // loads are pipelined: an iteration uses the score loaded one iteration
+    // earlier and the residues loaded two earlier (query reads clamped to the
+    // group's residues, subject reads wrapped in sbuf, score index kept in
+    // the 32 x 32 table: values past the walk are masked, never used)
+    auto q_at = [&](int32_t k) { return (uint32_t)Q[min((uint32_t)max(RIGHT ? cq + k : cq - k, 0), qcap - 1)]; };
+    auto s_at = [&](int32_t k) { return (uint32_t)sbuf[(uint32_t)(RIGHT ? cs + k : cs - k) & smask]; };
+    auto m_at = [&](uint32_t qb, uint32_t sb) { return (int32_t)M[((qb << 5) | sb) & 1023u]; };
+    int32_t m_next = m_at(q_at((int32_t)rl), s_at((int32_t)rl));
+    uint32_t q2 = q_at(W + (int32_t)rl), s2 = s_at(W + (int32_t)rl);
     for (int32_t k0 = 0; __ballot(!done) != 0; k0 += W) {
         const int32_t k = k0 + (int32_t)rl;
         const bool ok = !done && k < nsteps;
-        // left steps walk down from (cq, cs), right steps walk up from it
-        const int32_t qi = RIGHT ? cq + k : cq - k;
-        const int32_t si = RIGHT ? cs + k : cs - k;
-        const uint32_t qb = Q[min((uint32_t)max(qi, 0), qcap - 1)];
-        const uint32_t sb = sbuf[(uint32_t)max(si, 0) & smask];
-        const int32_t m = ok ? (int32_t)M[qb * kMatrixStride + sb] : kNeg;
+        const int32_t m_raw = m_next;
+        m_next = m_at(q2, s2);
+        q2 = q_at(k + 2 * W);
+        s2 = s_at(k + 2 * W);
+        const int32_t m = ok ? m_raw : kNeg;
         const int32_t run = carry + seg_incl_sum<W>(m, rl);
         const int32_t pm = max(mx, seg_incl_max<W>(run, rl));
         const bool stop = ok ? ((RIGHT && run <= 0) || pm - run >= xdrop) : true;
@@ -370,9 +490,9 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t* ctl = reinterpret_cast<uint32_t*>(lds);
 
     int8_t* M = reinterpret_cast<int8_t*>(lds + p.lds_matrix);
-    for (uint32_t i = tid; i < 28 * kMatrixStride; i += blockDim.x) {
+    for (uint32_t i = tid; i < kMatrixRows * kMatrixStride; i += blockDim.x) {
         const uint32_t r = i / kMatrixStride, c = i % kMatrixStride;
-        M[i] = c < 28 ? p.matrix[r * 28 + c] : (int8_t)-128;
+        M[i] = (r < 28 && c < 28) ? p.matrix[r * 28 + c] : (int8_t)-128;
     }
     uint32_t* tab_lds = reinterpret_cast<uint32_t*>(lds + p.lds_table);
     uint8_t* Q = lds + p.lds_qres;
@@ -716,8 +836,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                                                       (int32_t)qi.qlen, tsl, ts, tc, qi.xdrop, lane);
                     } else if (mine) {
                         if (resident)
-                            x = extend_two_hit<WORD, true>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
-                                                           sbuf, smask, S, 0, whi, slen, tsl, ts, tc, qi.xdrop);
+                            x = extend_resident<WORD>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, slen,
+                                                      tsl, ts, tc, qi.xdrop);
                         else
                             x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
                                                             sbuf, smask, S, whi - (int32_t)p.sbuf, whi, slen, tsl, ts,
@@ -1206,7 +1326,7 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     pl.qinfo = at;
     at += pl.max_nq * (uint32_t)sizeof(QueryInfo);
     pl.matrix = at;
-    at += up16(28 * kMatrixStride);
+    at += up16(kMatrixRows * kMatrixStride);
     pl.wave = at;
     at += pl.wave_stride * (uint32_t)waves;
     pl.total = at;
