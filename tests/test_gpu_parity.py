@@ -467,6 +467,21 @@ def test_batch_per_query_parameters_and_policies(ctx):
         assert_batch_parity(ctx, queries, db, xd, co, **pol)
 
 
+@pytest.mark.parametrize("launch", [dict(claims_per_share=1), dict(waves_per_group=1, claims_per_share=2),
+                                    dict(subjects_per_item=8)])
+def test_batch_multi_subject_claims_match_oracle(ctx, launch):
+    """Packed groups on a database large enough that a wavefront claims
+    several subjects at a time (claims_per_share 1 and 2 give chunks of up to
+    16), and with explicit 8-subject work items: every query equals the
+    oracle."""
+    rng = np.random.default_rng(25)
+    lens = [int(x) for x in rng.integers(30, 700, 24)]
+    queries = [bpx.synth_query(25, i, L) for i, L in enumerate(lens)]
+    db = batch_db(queries, n=30000, seed=35)
+    say(f"launch {launch}")
+    assert_batch_parity(ctx, queries, db, launch=launch, group_residues=2000, group_queries=8)
+
+
 def test_autotuned_batch_matches_oracle(ctx):
     """The autotuner only reads timings: whatever it picks, the batched scan
     under that packing and launch shape equals the oracle per query."""
