@@ -5,9 +5,12 @@
 // 42-320) gives each CUDA thread a whole subject with a private diagonal
 // array in global memory, so every word hit is a random global read-modify-
 // write and every extension step a dependent global load.  Here a 64-lane
-// wavefront owns one subject at a time:
+// wavefront owns a pack of subjects at a time -- one long subject, or up to
+// four consecutive ones that fit its subject buffer together, scanned as one
+// stream (each at NCBI's running diagonal offset from the previous one, so
+// the two-hit state is exactly the per-subject one):
 //
-//   * the subject is staged into a per-wave LDS buffer with 16-byte loads
+//   * the pack is staged into a per-wave LDS buffer with 16-byte loads
 //     (long subjects through a sliding window); words and extensions read it
 //     from LDS;
 //   * the workgroup stages the group's packed lookup table, its query
@@ -27,11 +30,11 @@
 //     deferred, in order, and applied again after the queue is flushed, so
 //     every slot still sees its hits strictly in scan order.  Queued
 //     extensions (s_BlastAaExtendTwoHit/Left/Right semantics,
-//     gpu_blastp_kernel.cu:341-549) run when a queue fills or the subject
+//     gpu_blastp_kernel.cu:341-549) run when a queue fills or the pack
 //     ends: one per lane, or a few on 8-lane segments;
-//   * records go to the wave's private output segment; per-(query, subject)
-//     counts, optional counters and per-query ungapped statistics are written
-//     when the subject is done.
+//   * records go to the wave's private output segment, tagged with their
+//     subject; per-(query, subject) counts, optional counters and per-query
+//     ungapped statistics are written when the pack is done.
 //
 // Records are put in (query, subject, subject offset, query offset) order by
 // the count / scan / gather kernels below (no atomics on the output).
@@ -236,12 +239,12 @@ __device__ __forceinline__ TwoHit extend_two_hit(const int8_t* M, const uint8_t*
 // general form with two, three or four steps per iteration measured slower.)
 template <int WORD>
 __device__ __forceinline__ TwoHit extend_resident(const int8_t* M, const uint8_t* Q, int32_t qstart, int32_t qlen,
-                                                  const uint8_t* sbuf, uint32_t smask, int32_t slen, int32_t s_left,
-                                                  int32_t s, int32_t c, int32_t xdrop) {
+                                                  const uint8_t* sbuf, uint32_t smask, uint32_t sbase, int32_t slen,
+                                                  int32_t s_left, int32_t s, int32_t c, int32_t xdrop) {
     int32_t sc = 0, best = 0, rd = 0;
 #pragma unroll
     for (int32_t i = 0; i < WORD; ++i) {
-        sc += M[Q[c + i] * kMatrixStride + sbuf[(uint32_t)(s + i) & smask]];
+        sc += M[Q[c + i] * kMatrixStride + sbuf[(sbase + (uint32_t)(s + i)) & smask]];
         if (sc > best) {
             best = sc;
             rd = i + 1;
@@ -257,7 +260,7 @@ __device__ __forceinline__ TwoHit extend_resident(const int8_t* M, const uint8_t
     int32_t mx = 0, left_len = 0;
     if (n >= 0) {
         const uint8_t* qp = Q + (cr - 1);
-        const uint32_t sp = (uint32_t)(sr - 1);
+        const uint32_t sp = sbase + (uint32_t)(sr - 1);
         int32_t m0 = score_at(qp[0], sbuf[sp & smask]);
         int32_t m1 = score_at(qp[-1], sbuf[(sp - 1u) & smask]);
         uint32_t q2 = qp[-2], q3 = qp[-3], s2 = sbuf[(sp - 2u) & smask], s3 = sbuf[(sp - 3u) & smask];
@@ -295,7 +298,7 @@ __device__ __forceinline__ TwoHit extend_resident(const int8_t* M, const uint8_t
         int32_t rmax = mx, best_k = -1, kstop = n2;
         if (n2 > 0) {
             const uint8_t* qp = Q + cr;
-            const uint32_t sp = (uint32_t)sr;
+            const uint32_t sp = sbase + (uint32_t)sr;
             int32_t m0 = score_at(qp[0], sbuf[sp & smask]);
             int32_t m1 = score_at(qp[1], sbuf[(sp + 1u) & smask]);
             uint32_t q2 = qp[2], q3 = qp[3], s2 = sbuf[(sp + 2u) & smask], s3 = sbuf[(sp + 3u) & smask];
@@ -381,7 +384,8 @@ __device__ __forceinline__ int32_t seg_incl_max(int32_t v, uint32_t rl) {
 // (-1 if it never rose above the start value) and the stop step.
 template <bool RIGHT, int W>
 __device__ __forceinline__ void seg_xdrop(const int8_t* M, const uint8_t* Q, uint32_t qcap, const uint8_t* sbuf,
-                                          uint32_t smask, int32_t cq, int32_t cs, int32_t nsteps, int32_t start,
+                                          uint32_t smask, uint32_t sbase, int32_t cq, int32_t cs, int32_t nsteps,
+                                          int32_t start,
                                           int32_t xdrop, uint32_t lane, int32_t& out_max, int32_t& out_best,
                                           int32_t& out_stop) {
     constexpr int32_t kNeg = -(1 << 22);
@@ -395,7 +399,7 @@ __device__ __forceinline__ void seg_xdrop(const int8_t* M, const uint8_t* Q, uin
     // group's residues, subject reads wrapped in sbuf, score index kept in
     // the 32 x 32 table: values past the walk are masked, never used)
     auto q_at = [&](int32_t k) { return (uint32_t)Q[min((uint32_t)max(RIGHT ? cq + k : cq - k, 0), qcap - 1)]; };
-    auto s_at = [&](int32_t k) { return (uint32_t)sbuf[(uint32_t)(RIGHT ? cs + k : cs - k) & smask]; };
+    auto s_at = [&](int32_t k) { return (uint32_t)sbuf[(sbase + (uint32_t)(RIGHT ? cs + k : cs - k)) & smask]; };
     auto m_at = [&](uint32_t qb, uint32_t sb) { return (int32_t)M[((qb << 5) | sb) & 1023u]; };
     int32_t m_next = m_at(q_at((int32_t)rl), s_at((int32_t)rl));
     uint32_t q2 = q_at(W + (int32_t)rl), s2 = s_at(W + (int32_t)rl);
@@ -437,13 +441,14 @@ __device__ __forceinline__ void seg_xdrop(const int8_t* M, const uint8_t* Q, uin
 // extend_two_hit for a subject resident in sbuf.
 template <int WORD, int W>
 __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uint8_t* Q, uint32_t qcap,
-                                                      const uint8_t* sbuf, uint32_t smask, int32_t slen, bool valid,
+                                                      const uint8_t* sbuf, uint32_t smask, uint32_t sbase,
+                                                      int32_t slen, bool valid,
                                                       int32_t qstart, int32_t qlen, int32_t s_left, int32_t s,
                                                       int32_t c, int32_t xdrop, uint32_t lane) {
     int32_t sc = 0, best = 0, rd = 0;
 #pragma unroll
     for (int32_t i = 0; i < WORD; ++i) {
-        sc += M[Q[min((uint32_t)(c + i), qcap - 1)] * kMatrixStride + sbuf[(uint32_t)(s + i) & smask]];
+        sc += M[Q[min((uint32_t)(c + i), qcap - 1)] * kMatrixStride + sbuf[(sbase + (uint32_t)(s + i)) & smask]];
         if (sc > best) {
             best = sc;
             rd = i + 1;
@@ -453,13 +458,13 @@ __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uin
     // left: steps k = 0..n at (cr-1-k, sr-1-k)
     const int32_t nl = valid ? min(sr - 1, qr - 1) + 1 : 0;
     int32_t lmax, lbest, lstop;
-    seg_xdrop<false, W>(M, Q, qcap, sbuf, smask, cr - 1, sr - 1, nl, 0, xdrop, lane, lmax, lbest, lstop);
+    seg_xdrop<false, W>(M, Q, qcap, sbuf, smask, sbase, cr - 1, sr - 1, nl, 0, xdrop, lane, lmax, lbest, lstop);
     const int32_t left_len = lbest + 1;
     TwoHit r;
     r.right_extend = valid && left_len >= sr - s_left;
     const int32_t nr = r.right_extend ? min(slen - sr, qlen - qr) : 0;
     int32_t rmax, rbest, rstop;
-    seg_xdrop<true, W>(M, Q, qcap, sbuf, smask, cr, sr, nr, lmax, xdrop, lane, rmax, rbest, rstop);
+    seg_xdrop<true, W>(M, Q, qcap, sbuf, smask, sbase, cr, sr, nr, lmax, xdrop, lane, rmax, rbest, rstop);
     r.s_last = r.right_extend ? sr + rstop : sr;
     r.hsp_q = qr - left_len;
     r.hsp_s = sr - left_len;
@@ -468,8 +473,15 @@ __device__ __forceinline__ TwoHit coop_extend_two_hit(const int8_t* M, const uin
     return r;
 }
 
-// Per-query counters of a wave (MULTI groups), 32 bytes each.
+// Per-query counters of a wave (MULTI groups), 16 bytes each.
 struct QCount {
+    uint32_t hits, ext, saved, pad;
+};
+// A wavefront scans a pack of up to kPack consecutive subjects that fit its
+// subject buffer together as one stream (see the pack loop in k_scan).
+constexpr int kPack = 4;
+// Per-subject counters of a pack (single-query groups).
+struct PackSub {
     uint32_t hits, ext, saved, pad;
 };
 // Per-query ungapped statistics of a work item (MULTI groups) live in the
@@ -511,7 +523,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t* t_sc = ring + p.qcap;
     uint16_t* t_s = reinterpret_cast<uint16_t*>(t_sc + kTaskCap);  // subject offset - qbase
     uint32_t* dq = reinterpret_cast<uint32_t*>(t_s + kTaskCap);
-    QCount* qc = reinterpret_cast<QCount*>(dq + kDefCap);
+    PackSub* pk = reinterpret_cast<PackSub*>(dq + kDefCap);  // this pack's subjects
+    QCount* qc = reinterpret_cast<QCount*>(pk + kPack);        // MULTI: kPack x max_nq
     // zero-fills the wave's diagonal table, 16 bytes per lane and store
     auto clear_diag = [&]() {
         uint4* d4 = reinterpret_cast<uint4*>(diag);
@@ -520,7 +533,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     clear_diag();
     for (uint32_t i = lane; i < p.claim_slots; i += 64) claim[i] = 0xFFFFFFFFu;
     if (MULTI)
-        for (uint32_t i = lane; i < p.max_nq; i += 64) qc[i] = QCount{};
+        for (uint32_t i = lane; i < p.max_nq * kPack; i += 64) qc[i] = QCount{};
+    if (lane < kPack) pk[lane] = PackSub{};
 
     const uint32_t smask = p.sbuf - 1;
     const uint32_t qcap = p.qcap, qmask = qcap - 1;
@@ -665,28 +679,109 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             const uint32_t slot = sid - p.first;
             const uint8_t* __restrict__ S = p.residues + soff;
             const int32_t send = (slen + 15) & ~15;
+            const bool resident = send <= (int32_t)p.sbuf;
 
-            // ---- stage the subject (or its first window) into LDS ----
+            // ---- the pack: this subject and, while they fit the subject
+            // buffer beside it, the next ones of the claimed chunk ----
+            // The wave scans them as one stream: subject t sits at buffer
+            // offset V_t (16-byte aligned, at least length + window past the
+            // previous subject's start), and a hit of subject t at s is a
+            // hit at V_t + s.  V_t - V_{t-1} >= length + window is NCBI's
+            // running diagonal offset between two subjects
+            // (Blast_ExtendWordExit), so every slot an earlier subject left
+            // reads as stale, exactly as after the per-subject reset.  The
+            // ring, the extension queue and the deferred hits carry across
+            // the subjects of a pack; its extensions run in shared flushes.
+            int32_t V1 = INT32_MAX, V2 = INT32_MAX, V3 = INT32_MAX;  // offsets (V_0 = 0)
+            int32_t N1 = 0, N2 = 0, N3 = 0;                         // lengths
+            uint32_t Z1 = 0, Z2 = 0, Z3 = 0;                        // slots
+            uint64_t O1 = 0, O2 = 0, O3 = 0;                        // arena offsets
+            int32_t npack = 1, vend = slen;                         // subjects, stream end
+            if (p.flat && resident) {
+                while (npack < kPack && c_next < c_end && c_next < w.count) {
+                    const int j = (int)(c_next - c_first);  // wave-uniform
+                    const int32_t ln = __builtin_amdgcn_readlane((int)d_len, j);
+                    const int32_t v = (vend + window + 15) & ~15;
+                    if (v + ((ln + 15) & ~15) > (int32_t)p.sbuf) break;
+                    const uint32_t z = (uint32_t)__builtin_amdgcn_readlane((int)d_sid, j) - p.first;
+                    const uint64_t o =
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)d_off, j) |
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(d_off >> 32), j) << 32);
+                    if (npack == 1) {
+                        V1 = v; N1 = ln; Z1 = z; O1 = o;
+                    } else if (npack == 2) {
+                        V2 = v; N2 = ln; Z2 = z; O2 = o;
+                    } else {
+                        V3 = v; N3 = ln; Z3 = z; O3 = o;
+                    }
+                    vend = v + ln;
+                    ++npack;
+                    ++c_next;
+                }
+            }
+            // the pack subject of stream position v
+            auto sub_of = [&](int32_t v) { return (int32_t)(v >= V1) + (int32_t)(v >= V2) + (int32_t)(v >= V3); };
+
+            // ---- stage the pack (or a long subject's first window) into LDS ----
             auto fill = [&](int32_t from, int32_t to) {
                 for (int32_t x = from + 16 * (int32_t)lane; x < to; x += 1024)
                     *reinterpret_cast<uint4*>(sbuf + (x & smask)) = *reinterpret_cast<const uint4*>(S + x);
             };
-            if (NARROW && doff + slen + window > (int32_t)kVal) {
+            if (NARROW && doff + vend + window > (int32_t)kVal) {
                 clear_diag();
                 doff = window;
             }
-            const bool resident = send <= (int32_t)p.sbuf;
             int32_t whi = min(send, (int32_t)p.sbuf);
-            fill(0, whi);
+            uint64_t staged = (uint64_t)whi;
+            if (npack == 1) {
+                fill(0, whi);
+            } else {
+                // every subject's first 1 KB is loaded before any is stored,
+                // so the pack's loads are in flight together
+                const uint8_t* __restrict__ S1 = p.residues + O1;
+                const uint8_t* __restrict__ S2 = p.residues + O2;
+                const uint8_t* __restrict__ S3 = p.residues + O3;
+                const int32_t e1 = (N1 + 15) & ~15, e2 = (N2 + 15) & ~15, e3 = (N3 + 15) & ~15;
+                const int32_t x = 16 * (int32_t)lane;
+                uint4 r0{}, r1{}, r2{}, r3{};
+                if (x < send) r0 = *reinterpret_cast<const uint4*>(S + x);
+                if (x < e1) r1 = *reinterpret_cast<const uint4*>(S1 + x);
+                if (x < e2) r2 = *reinterpret_cast<const uint4*>(S2 + x);
+                if (x < e3) r3 = *reinterpret_cast<const uint4*>(S3 + x);
+                if (x < send) *reinterpret_cast<uint4*>(sbuf + x) = r0;
+                if (x < e1) *reinterpret_cast<uint4*>(sbuf + V1 + x) = r1;
+                if (x < e2) *reinterpret_cast<uint4*>(sbuf + V2 + x) = r2;
+                if (x < e3) *reinterpret_cast<uint4*>(sbuf + V3 + x) = r3;
+                for (int32_t y = x + 1024; y < send; y += 1024)
+                    *reinterpret_cast<uint4*>(sbuf + y) = *reinterpret_cast<const uint4*>(S + y);
+                for (int32_t y = x + 1024; y < e1; y += 1024)
+                    *reinterpret_cast<uint4*>(sbuf + V1 + y) = *reinterpret_cast<const uint4*>(S1 + y);
+                for (int32_t y = x + 1024; y < e2; y += 1024)
+                    *reinterpret_cast<uint4*>(sbuf + V2 + y) = *reinterpret_cast<const uint4*>(S2 + y);
+                for (int32_t y = x + 1024; y < e3; y += 1024)
+                    *reinterpret_cast<uint4*>(sbuf + V3 + y) = *reinterpret_cast<const uint4*>(S3 + y);
+                staged += (uint64_t)(e1 + e2 + e3);
+                // the gaps between the pack's subjects hold code 31, so no
+                // word that reaches into a gap is present in the table
+                auto gap = [&](int32_t from, int32_t to) {
+                    for (int32_t y = from + (int32_t)lane; y < to; y += 64) sbuf[y] = (uint8_t)31;
+                };
+                gap(slen, V1);
+                if (npack > 2) gap(V1 + N1, V2);
+                if (npack > 3) gap(V2 + N2, V3);
+            }
             wave_sync();
             if (timed) {
                 t_stage += __builtin_amdgcn_s_memrealtime() - t_st0;
-                n_staged += 1;
-                b_staged += (uint64_t)whi;
+                n_staged += (uint64_t)npack;
+                b_staged += staged;
             }
 
-            uint32_t hits = 0, extended = 0, saved = 0;   // single-query counters
-            const uint32_t rec_start = seg_fill;
+            // single-query word hits, accumulated for pack subject hsub and
+            // stored to pk[] when the scan passes into the next subject
+            uint32_t hits_cur = 0;
+            int32_t hsub = 0;
+            const uint32_t rec_start = seg_fill;  // the pack's run of records
             uint32_t head = 0, count = 0;                 // hit ring (wave-uniform)
             uint32_t ntasks = 0, ndef = 0;                // queued extensions, deferred hits
             uint32_t rep = 0, rep_total = 0;              // deferred hits being re-applied
@@ -735,7 +830,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     dbase = qi.dbase;
                     dmask = qi.dmask;
                     qstart = (int32_t)qi.qstart;
-                    if (active && fresh) atomicAdd(&qc[j].hits, 1u);
+                    if (active && fresh) atomicAdd(&qc[sub_of(s) * p.max_nq + j].hits, 1u);
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
                 // The first round hashes the slot by its low bits; later rounds
@@ -826,19 +921,27 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     const uint32_t tslot = sc & 0xFFFFu;
                     int32_t tc = (int32_t)(sc >> 16);
                     const uint32_t tj = MULTI ? (uint32_t)qmap[tc] : 0u;
-                    const int32_t ts = qbase + (int32_t)t_s[ti];
+                    const int32_t tv_s = qbase + (int32_t)t_s[ti];  // stream position of the second hit
+                    // its pack subject: buffer offset, length, slot; the
+                    // extension runs in that subject's own coordinates
+                    const int32_t tp = sub_of(tv_s);
+                    const int32_t tv = tp == 0 ? 0 : (tp == 1 ? V1 : (tp == 2 ? V2 : V3));
+                    const int32_t tn = tp == 0 ? slen : (tp == 1 ? N1 : (tp == 2 ? N2 : N3));
+                    const uint32_t tz = tp == 0 ? slot : (tp == 1 ? Z1 : (tp == 2 ? Z2 : Z3));
+                    const int32_t ts = tv_s - tv;
                     // left boundary: the first hit (kept in the slot) + word
-                    const int32_t tsl = (int32_t)(diag[tslot] & kVal) - doff + WORD;
+                    const int32_t tsl = (int32_t)(diag[tslot] & kVal) - doff - tv + WORD;
                     const QueryInfo qi = MULTI ? qinfo[tj] : q0;
                     TwoHit x{};
                     if (coop) {
-                        x = coop_extend_two_hit<WORD, kSeg>(M, Q, p.qres_cap, sbuf, smask, slen, tvalid, (int32_t)qi.qstart,
-                                                      (int32_t)qi.qlen, tsl, ts, tc, qi.xdrop, lane);
+                        x = coop_extend_two_hit<WORD, kSeg>(M, Q, p.qres_cap, sbuf, smask, (uint32_t)tv, tn, tvalid,
+                                                            (int32_t)qi.qstart, (int32_t)qi.qlen, tsl, ts, tc,
+                                                            qi.xdrop, lane);
                     } else if (mine) {
                         if (resident)
-                            x = extend_resident<WORD>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask, slen,
-                                                      tsl, ts, tc, qi.xdrop);
-                        else
+                            x = extend_resident<WORD>(M, Q, (int32_t)qi.qstart, (int32_t)qi.qlen, sbuf, smask,
+                                                      (uint32_t)tv, tn, tsl, ts, tc, qi.xdrop);
+                        else  // a long subject: always alone in its pack
                             x = extend_two_hit<WORD, false>(M, Q, p.qres_cap, (int32_t)qi.qstart, (int32_t)qi.qlen,
                                                             sbuf, smask, S, whi - (int32_t)p.sbuf, whi, slen, tsl, ts,
                                                             tc, qi.xdrop);
@@ -846,18 +949,33 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     bool rec = false;
                     if (mine) {
                         rec = x.score >= qi.cutoff;
-                        diag[tslot] = (SlotT)(x.right_extend ? ((uint32_t)(x.s_last - (WORD - 1) + doff) | kFlag)
-                                                             : (uint32_t)(ts + doff));
+                        diag[tslot] = (SlotT)(x.right_extend ? ((uint32_t)(x.s_last + tv - (WORD - 1) + doff) | kFlag)
+                                                             : (uint32_t)(tv_s + doff));
                         if (MULTI) {
-                            atomicAdd(&qc[tj].ext, 1u);
-                            if (rec) atomicAdd(&qc[tj].saved, 1u);
+                            QCount& cq = qc[tp * p.max_nq + tj];
+                            atomicAdd(&cq.ext, 1u);
+                            if (rec) atomicAdd(&cq.saved, 1u);
                         }
                         tc -= (int32_t)qi.qstart;  // query-local offset of the second hit
                     }
                     const unsigned long long rb = __ballot(rec);
                     if (!MULTI) {
-                        extended += (uint32_t)__popcll(__ballot(mine));
-                        saved += (uint32_t)__popcll(rb);
+                        // per pack subject: extensions run and records kept
+                        if (npack == 1) {
+                            const unsigned long long mb = __ballot(mine);
+                            if (lane == 0 && mb) {
+                                pk[0].ext += (uint32_t)__popcll(mb);
+                                pk[0].saved += (uint32_t)__popcll(rb);
+                            }
+                        } else {
+                            for (int u = 0; u < npack; ++u) {
+                                const unsigned long long mi = __ballot(mine && tp == u);
+                                if (lane == 0 && mi) {
+                                    pk[u].ext += (uint32_t)__popcll(mi);
+                                    pk[u].saved += (uint32_t)__popcll(rb & mi);
+                                }
+                            }
+                        }
                     }
                     if (rb) {
                         const uint32_t at = seg_fill + rank_below(rb);
@@ -871,7 +989,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                                 r.h.hsp_len = x.len;
                                 r.h.score = x.score;
                                 r.j = tj;
-                                r.pad = 0;
+                                r.slot = tz;
                                 p.seg[seg_base + at] = r;
                             } else {
                                 atomicOr(p.overflow, 1u);
@@ -895,7 +1013,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // end, 16-bit offset rebase); apply the next ring batch (a full
             // batch is ready, or the ring must be emptied); fill the ring
             // from the next 64 positions.
-            const int32_t last = slen - WORD;
+            const int32_t last = vend - WORD;  // the pack's last word position
             int32_t pos = 0;
             bool empty_ring = false, full_drain = false;
 
@@ -921,6 +1039,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // never have been written (any value): keep their word in
                 // range so its presence-word read stays inside the table
                 wd &= (1u << (5 * WORD)) - 1u;
+                // a word holding a pack's gap byte (code 31) is absent: as a
+                // later residue its presence bit is never set (no table word
+                // holds code 31); as the first, the clamp below lands on the
+                // presence word of (27, 31, *) for 3-letter words, also
+                // empty; 2-letter words would land on (27, *), so they are
+                // sent to (27, 31)
+                if (WORD == 2 && L.b[0] >= 28u) wd = (27u << 5) | 31u;
                 L.wd = wd;
                 L.gv = pvr[min(wd >> 5, p.ngroups_pv - 1u)];
             };
@@ -980,7 +1105,23 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // nfit is wave-uniform: a lane read, not an LDS permute
                 const uint32_t added = (uint32_t)__builtin_amdgcn_readlane((int)incl, (int)nfit - 1);
                 count += added;
-                hits += added;
+                if (!MULTI) {
+                    // the hits of lanes past a subject boundary belong to the
+                    // next subject(s) of the pack
+                    uint32_t taken = 0;
+                    for (;;) {
+                        const int32_t vb = hsub == 0 ? V1 : (hsub == 1 ? V2 : (hsub == 2 ? V3 : INT32_MAX));
+                        if (pos + (int32_t)nfit <= vb) break;
+                        const int32_t lb = max(vb - pos, 0);  // lanes before the boundary
+                        const uint32_t h = lb > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)incl, lb - 1) : taken;
+                        hits_cur += h - taken;
+                        if (lane == 0) pk[hsub].hits = hits_cur;
+                        taken = h;
+                        hits_cur = 0;
+                        ++hsub;
+                    }
+                    hits_cur += added - taken;
+                }
                 pos += (int32_t)nfit;
                 wave_sync();
             };
@@ -1051,58 +1192,73 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 }
             }
 
-            // ---- per-subject outputs ----
+            // ---- the pack's outputs ----
             // Counts and run lengths were zero-filled before the launch, so
             // only subjects with records write them.  The per-subject
             // counters go straight to pinned host memory at the subject's
             // slot: 12-byte posted writes over the host link while the scan
-            // runs, instead of a permute and a copy after it.
+            // runs, instead of a permute and a copy after it.  The pack's
+            // records form one run, filed under its first subject; the gather
+            // sorts it by subject.
             const uint32_t gslot = w.group * p.count + slot;
             const bool has_run = seg_fill > rec_start;
             if (!MULTI) {
-                if (lane == 0) {
-                    if (saved) p.cnt[(size_t)q0.qid * p.count + slot] = saved;
-                    if (p.stats) {
-                        int32_t* st = p.stats + 3 * ((size_t)q0.qid * p.count + slot);
-                        st[0] = (int32_t)hits;
-                        st[1] = (int32_t)saved;
-                        st[2] = (int32_t)extended;
+                if (lane == 0) pk[hsub].hits = hits_cur;
+                wave_sync();
+                for (int u = 0; u < npack; ++u) {
+                    const PackSub c = pk[u];
+                    const uint32_t z = u == 0 ? slot : (u == 1 ? Z1 : (u == 2 ? Z2 : Z3));
+                    if (lane == 0) {
+                        if (c.saved) p.cnt[(size_t)q0.qid * p.count + z] = c.saved;
+                        if (p.stats) {
+                            int32_t* st = p.stats + 3 * ((size_t)q0.qid * p.count + z);
+                            st[0] = (int32_t)c.hits;
+                            st[1] = (int32_t)c.saved;
+                            st[2] = (int32_t)c.ext;
+                        }
                     }
-                    if (has_run) {
-                        p.run_loc[gslot] = seg_base + rec_start;
-                        p.run_len[gslot] = seg_fill - rec_start;
+                    if (c.hits > 0) {  // Blast_UngappedStatsUpdate skips subjects without hits
+                        a_hits += c.hits;
+                        a_nseq += 1;
+                        a_ext += c.ext;
+                        a_saved += c.saved;
+                        a_passed += c.saved > 0;
                     }
                 }
-                if (hits > 0) {  // Blast_UngappedStatsUpdate skips subjects without hits
-                    a_hits += hits;
-                    a_nseq += 1;
-                    a_ext += extended;
-                    a_saved += saved;
-                    a_passed += saved > 0;
+                wave_sync();
+                if (lane < (uint32_t)npack) pk[lane] = PackSub{};
+                if (lane == 0 && has_run) {
+                    p.run_loc[gslot] = seg_base + rec_start;
+                    p.run_len[gslot] = seg_fill - rec_start;
                 }
+                wave_sync();
             } else {
                 wave_sync();
+                for (int u = 0; u < npack; ++u) {
+                    const uint32_t z = u == 0 ? slot : (u == 1 ? Z1 : (u == 2 ? Z2 : Z3));
+                    QCount* qe = qc + u * p.max_nq;
 #pragma unroll
-                for (int t = 0; t < kQuerySlots; ++t) {
-                    const uint32_t jj = lane + 64u * (uint32_t)t;
-                    if (jj < g.nq) {
-                        QCount& cq = qc[jj];
-                        const size_t qrow = (size_t)qinfo[jj].qid * p.count;
-                        if (cq.saved) p.cnt[qrow + slot] = cq.saved;
-                        if (p.stats) {
-                            int32_t* st = p.stats + 3 * (qrow + slot);
-                            st[0] = (int32_t)cq.hits;
-                            st[1] = (int32_t)cq.saved;
-                            st[2] = (int32_t)cq.ext;
+                    for (int t = 0; t < kQuerySlots; ++t) {
+                        const uint32_t jj = lane + 64u * (uint32_t)t;
+                        if (jj < g.nq) {
+                            QCount& cq = qe[jj];
+                            const size_t qrow = (size_t)qinfo[jj].qid * p.count;
+                            if (cq.saved) p.cnt[qrow + z] = cq.saved;
+                            if (p.stats) {
+                                int32_t* st = p.stats + 3 * (qrow + z);
+                                st[0] = (int32_t)cq.hits;
+                                st[1] = (int32_t)cq.saved;
+                                st[2] = (int32_t)cq.ext;
+                            }
+                            if (cq.hits > 0) {
+                                m_hits[t] += cq.hits;
+                                m_nseq[t] += 1;
+                                m_ext[t] += cq.ext;
+                                m_saved[t] += cq.saved;
+                                m_passed[t] += cq.saved > 0;
+                            }
+                            cq.hits = cq.ext = cq.saved = 0;
                         }
-                        if (cq.hits > 0) {
-                            m_hits[t] += cq.hits;
-                            m_nseq[t] += 1;
-                            m_ext[t] += cq.ext;
-                            m_saved[t] += cq.saved;
-                            m_passed[t] += cq.saved > 0;
-                        }
-                        cq.hits = cq.ext = cq.saved = 0;
                     }
                 }
                 if (lane == 0 && has_run) {
@@ -1112,7 +1268,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 wave_sync();
             }
             // Blast_ExtendWordExit: a fresh diagonal range for the next
-            // subject -- NCBI's running offset, advanced by the subject's
+            // pack -- NCBI's running offset, advanced by the pack's stream
             // length + window, so every slot a previous subject left reads as
             // a hit at least a window back (or a flag behind the next hit).
             // The table is cleared only when the offset would outgrow the
@@ -1123,7 +1279,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 doff = window;
                 wave_sync();
             } else {
-                doff += slen + window;
+                doff += vend + window;
             }
         }
 
@@ -1221,6 +1377,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_offsets(CompactParams p) {
 // Record order within a (group, subject) run: query, subject offset, query
 // offset.  Keys are unique (a second hit gives at most one record per query).
 __device__ __forceinline__ bool rec_before(const SegRecord& a, const SegRecord& b) {
+    if (a.slot != b.slot) return a.slot < b.slot;
     return a.j < b.j || (a.j == b.j && (a.h.subject_offset < b.h.subject_offset ||
                                         (a.h.subject_offset == b.h.subject_offset &&
                                          a.h.query_offset < b.h.query_offset)));
@@ -1263,7 +1420,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
     if (i >= (uint64_t)p.ngroups * p.count) return;
     const uint32_t len = p.run_len[i];
     if (len == 0) return;
-    const uint32_t g = (uint32_t)(i / p.count), slot = (uint32_t)(i % p.count);
+    const uint32_t g = (uint32_t)(i / p.count);
     SegRecord* run = p.seg + p.run_loc[i];
     if (len > kInsertionMax) {
         heap_sort_run(run, len);
@@ -1278,18 +1435,21 @@ __global__ __launch_bounds__(kCompactThreads) void k_gather_runs(CompactParams p
             run[m] = r;
         }
     }
+    // a run holds the records of one or more consecutive subjects (a
+    // wavefront's pack), now in (subject, query, s_off, q_off) order
     const GroupInfo gi = p.groups[g];
-    uint32_t cur_j = 0xFFFFFFFFu, dst = 0, qid = 0;
+    uint32_t cur_j = 0xFFFFFFFFu, cur_slot = 0xFFFFFFFFu, dst = 0, qid = 0;
     for (uint32_t a = 0; a < len; ++a) {
         const SegRecord r = run[a];
-        if (r.j != cur_j) {
+        if (r.j != cur_j || r.slot != cur_slot) {
             cur_j = r.j;
+            cur_slot = r.slot;
             qid = p.qinfo[gi.qinfo_first + r.j].qid;
-            dst = p.cnt[(uint64_t)qid * p.count + slot];
+            dst = p.cnt[(uint64_t)qid * p.count + r.slot];
         }
         if (dst < p.rec_cap) {
             p.records[dst] = r.h;
-            p.subjects[dst] = p.first + slot;
+            p.subjects[dst] = p.first + r.slot;
             if (p.queries) p.queries[dst] = qid;
         }
         ++dst;
@@ -1313,8 +1473,9 @@ LdsPlan plan_lds(uint32_t max_table_bytes, uint32_t max_qres, uint32_t max_nq, u
     pl.qcap = qcap;
     pl.claim_slots = claim_slots;
     pl.qres_cap = up16(max_qres < 16 ? 16 : max_qres);
-    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + claim_slots * 4 + qcap * 4 + kTaskCap * 6 + kDefCap * 4 +
-                     (pl.max_nq > 1 ? pl.max_nq * (uint32_t)sizeof(QCount) : 0u);
+    pl.wave_stride = up16(sbuf) + up16(diag_cap * (narrow ? 2u : 4u)) + claim_slots * 4 + qcap * 4 + kTaskCap * 6 +
+                     kDefCap * 4 + kPack * (uint32_t)sizeof(PackSub) +
+                     (pl.max_nq > 1 ? pl.max_nq * kPack * (uint32_t)sizeof(QCount) : 0u);
     uint32_t at = kCtlBytes;
     pl.table = at;
     pl.table_cap_words = table_in_lds ? up16(max_table_bytes) / 4 : 0u;

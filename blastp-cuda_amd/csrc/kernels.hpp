@@ -46,12 +46,12 @@ struct WorkItem {
     uint32_t chunk;   // flat launches: subjects a wavefront claims at a time (0: the launch's)
 };
 
-// Record as staged in a wavefront's output segment: the HSP plus the index
-// of its query within the group.
+// Record as staged in a wavefront's output segment: the HSP, the index of
+// its query within the group and its subject's slot (subject id - first).
 struct SegRecord {
     bpx_init_hsp h;
     uint32_t j;
-    uint32_t pad;
+    uint32_t slot;
 };
 
 struct ScanParams {
