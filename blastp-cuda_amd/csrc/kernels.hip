@@ -902,11 +902,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 wave_sync();
             };
 
-            // Runs the queued extensions and writes their outcome to the
-            // diagonal table.  A few tasks run on kSeg-lane segments
+            // Runs the first `take` queued extensions and writes their
+            // outcome to the diagonal table; the others move to the front of
+            // the queue, in order (their slots stay pending, and hits on them
+            // stay deferred).  A few tasks run on kSeg-lane segments
             // (64 / kSeg per pass); more run one per lane.
-            auto flush = [&]() {
-                const uint32_t nt = ntasks;
+            auto flush = [&](uint32_t take) {
+                const uint32_t nt = take;
                 const uint64_t t_f0 = timed ? __builtin_amdgcn_s_memrealtime() : 0;
                 wave_sync();
                 constexpr uint32_t kPer = 64u / kSeg;
@@ -998,7 +1000,23 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         seg_fill += (uint32_t)__popcll(rb);
                     }
                 }
-                ntasks = 0;
+                // the rest (at most 64: take = 64 < ntasks <= kTaskCap) to
+                // the front; the source [nt, ntasks) lies past the
+                // destination [0, ntasks - nt) since nt >= ntasks - nt
+                const uint32_t rest = ntasks - nt;
+                if (rest) {
+                    uint32_t a = 0, b = 0;
+                    if (lane < rest) {
+                        a = t_sc[nt + lane];
+                        b = t_s[nt + lane];
+                    }
+                    wave_sync();
+                    if (lane < rest) {
+                        t_sc[lane] = a;
+                        t_s[lane] = (uint16_t)b;
+                    }
+                }
+                ntasks = rest;
                 wave_sync();
                 if (timed) {
                     t_flush += __builtin_amdgcn_s_memrealtime() - t_f0;
@@ -1138,7 +1156,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     n = min(count, 64u);
                     const bool batch = count >= 64 || ((empty_ring || drain_all) && count > 0);
                     if (batch ? (ntasks + n > kTaskCap || ndef + n > kDefCap) : (drain_all && ntasks > 0)) {
-                        flush();
+                        // a queue-full flush runs one full pass of 64 and
+                        // keeps the rest queued (a second, partly idle pass
+                        // costs about as much as a full one), unless the
+                        // kept tasks and one task per deferred hit might not
+                        // fit the queue when the deferred hits are re-applied
+                        flush(!drain_all && ntasks > 64u && ntasks - 64u + ndef <= kTaskCap ? 64u : ntasks);
                         rep = 0;
                         rep_total = ndef;
                         ndef = 0;

@@ -116,8 +116,9 @@ struct LdsPlan {
 
 constexpr uint32_t kTaskCap = 128;     // queued extensions per wave
 constexpr uint32_t kDefCap = 128;      // deferred hits per wave (hits on pending slots)
-// Deferred hits are re-applied right after a flush empties the extension
-// queue; each can queue at most one extension, so they always fit.
+// Deferred hits are re-applied right after a flush; each can queue at most
+// one extension, and a flush keeps at most kTaskCap minus that many tasks
+// queued, so they always fit.
 static_assert(kDefCap <= kTaskCap, "re-applied deferred hits must fit the emptied extension queue");
 
 // Narrow launches keep 16-bit diagonal slots and clear the table for every
