@@ -830,7 +830,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     dbase = qi.dbase;
                     dmask = qi.dmask;
                     qstart = (int32_t)qi.qstart;
-                    if (active && fresh) atomicAdd(&qc[sub_of(s) * p.max_nq + j].hits, 1u);
+                    if (active && fresh) atomicAdd(&qc[__umul24((uint32_t)sub_of(s), p.max_nq) + j].hits, 1u);
                 }
                 const uint32_t dslot = dbase + ((uint32_t)((c - qstart) - s) & dmask);
                 // The first round hashes the slot by its low bits; later rounds
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // every round's hash, so the lowest still goes first.
                 const uint32_t cmask = p.claim_slots - 1;
                 uint32_t h = dslot & cmask;
-                const uint32_t h2 = ((dslot * 0x9E3779B1u) >> 16) & cmask;
+                const uint32_t h2 = (__umul24(dslot, 0x9E3779u) >> 16) & cmask;  // 24-bit multiply: full rate
                 bool deferred = false;
                 const int32_t sd = s + doff;
                 for (int round = 0; (kRounds == 0 || round < kRounds) && __ballot(active) != 0; ++round) {
@@ -864,8 +864,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                     // the extension's end; unflagged: a hit beyond the window
                     // restarts it, one at least a word past the last completes
                     // a pair
-                    const bool restart = flagged ? diff >= 0 : diff >= window;
-                    const bool task = live && !flagged && !restart && diff >= WORD;
+                    const bool restart = diff >= (flagged ? 0 : window);
+                    const bool task = live && !flagged && diff < window && diff >= WORD;
                     const bool write = live && (restart || task);
                     const uint32_t ns = task ? (st | kPend) : (uint32_t)sd;
                     deferred = deferred || (win && !live);
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         diag[tslot] = (SlotT)(x.right_extend ? ((uint32_t)(x.s_last + tv - (WORD - 1) + doff) | kFlag)
                                                              : (uint32_t)(tv_s + doff));
                         if (MULTI) {
-                            QCount& cq = qc[tp * p.max_nq + tj];
+                            QCount& cq = qc[__umul24((uint32_t)tp, p.max_nq) + tj];
                             atomicAdd(&cq.ext, 1u);
                             if (rec) atomicAdd(&cq.saved, 1u);
                         }
