@@ -92,6 +92,27 @@ def measured_traffic(name, policy=None):
     return t.get("bytes_per_launch"), f"profiles/{name}.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
 
 
+def phase_summary(ph):
+    """Per-wavefront phase times of one scan (bpx_result_phases, measured
+    outside the timed steps).  The subject-streaming phase -- descriptor
+    reads and subject residues copied from HBM into LDS -- takes
+    `stage_share` of the wavefronts' time; applied to the launch span that is
+    `stream_phase_ms`, over which it moves `bytes_staged`: its rate against
+    the HBM peak.  `flush_share` is the share of the wavefronts' time in
+    extension flushes; 1 - `busy_fraction` the share of the launches'
+    wavefront slots left idle by wavefronts that finished before the
+    slowest (what any reordering of the work could at most recover)."""
+    share = ph["stage_ms_mean"] / ph["wave_ms_mean"] if ph["wave_ms_mean"] else 0.0
+    ms = share * ph["span_ms"]
+    gbs = ph["bytes_staged"] / (ms / 1e3) / 1e9 if ms else 0.0
+    return {"stage_share": round(share, 4), "stream_phase_ms": round(ms, 3), "bytes_staged": int(ph["bytes_staged"]),
+            "subjects_staged": int(ph["subjects_staged"]), "stream_phase_gbs": round(gbs, 1),
+            "stream_phase_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "flush_share": round(ph["flush_ms_mean"] / ph["wave_ms_mean"], 4) if ph["wave_ms_mean"] else 0.0,
+            "busy_fraction": round(ph["busy_fraction"], 5), "span_ms": round(ph["span_ms"], 3),
+            "launch_classes": int(ph["classes"]), "waves": int(ph["waves"])}
+
+
 def median_rate(fn, work, runs):
     """work / median wall time of `runs` calls of fn (after one warm-up)."""
     fn()
@@ -211,12 +232,16 @@ def plain_leg(ctx, threads, steps, warmup):
         kernel_ms.append(t["kernel_ms"])
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     kern = float(np.mean(kernel_ms))
+    # per-wavefront phase times of one more scan, outside the timed steps
+    r = bpx.scan(ctx, dq, ddb, subject_stats=True, phases=True).wait()
+    phases = phase_summary(r.phases())
+    r.close()
     # algorithmic HBM bytes of the launch: residues (1 B) + descriptors
     # (12 B) read, records (24 B) written; the per-subject counters (12 B)
     # go to pinned host memory
     bytes_alg = db.num_residues + 12 * db.num_subjects + 24 * nrec
     out = dict(q=q, db=db, ddb=ddb, dq=dq, ms=ms, kern=kern, kernel_ms=kernel_ms, nrec=nrec, ust=ust,
-               load_s=load_s, bytes_alg=bytes_alg, value=db.num_residues / (ms / 1000.0))
+               load_s=load_s, bytes_alg=bytes_alg, value=db.num_residues / (ms / 1000.0), phases=phases)
     return out
 
 
@@ -245,7 +270,8 @@ def plain(args):
                       "queries": 1, "query_len": 500, "subjects": db.num_subjects, "residues": db.num_residues,
                       "db_sha256": db.sha256, "records": int(P["nrec"]), "lookup_hits": P["ust"]["lookup_hits"],
                       "init_extends": P["ust"]["init_extends"], "db_load_s": round(P["load_s"], 4),
-                      "cold_search_residues_per_s": round(db.num_residues / (P["load_s"] + P["ms"] / 1000.0), 1)},
+                      "cold_search_residues_per_s": round(db.num_residues / (P["load_s"] + P["ms"] / 1000.0), 1),
+                      "phases": P["phases"]},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -335,6 +361,11 @@ def headline(args):
         f"{np.mean(kernel_ms):.1f} ms)")
     if ms_lf is not None and n_lf != nrec:
         raise SystemExit(f"record counts differ between orders: {n_lf} vs {nrec}")
+    # per-wavefront phase times of one more (eager) scan in the learned order
+    r = bpx.scan_batch(ctx, b, ddb, launch=launch, phases=True).wait()
+    batched_phases = phase_summary(r.phases())
+    r.close()
+    log(f"  batched phases: {batched_phases}")
 
     if args.only_batched:
         print(json.dumps({"workload": "batched only (profiling run)", "ms_per_step": round(ms, 3),
@@ -374,7 +405,8 @@ def headline(args):
               "db_sha256": P["db"].sha256, "kernel_ms_mean": round(P["kern"], 4),
               "roofline_frac": round(P["bytes_alg"] / (P["kern"] / 1000.0) / 1e9 / HBM_PEAK_GBS, 5),
               "db_load_s": round(P["load_s"], 4),
-              "cold_search_residues_per_s": round(P["db"].num_residues / (P["load_s"] + P["ms"] / 1000.0), 1)}
+              "cold_search_residues_per_s": round(P["db"].num_residues / (P["load_s"] + P["ms"] / 1000.0), 1),
+              "phases": P["phases"]}
     P["dq"].close()
     P["ddb"].close()
 
@@ -411,7 +443,7 @@ def headline(args):
                       "ms_longest_first": round(ms_lf, 3), "steps_longest_first": lf_steps,
                       "gain_over_longest_first": round(ms_lf / ms, 4),
                       "cost_model_train_s": round(train_s, 2), "cost_model_inference_s": round(infer_s, 3),
-                      "plain_2M": plain2},
+                      "phases": batched_phases, "plain_2M": plain2},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -507,15 +539,18 @@ def skewed(args):
         n = g.summary()[0]
         kern = g.timing()["kernel_ms"]
         g.close()
-        log(f"  {label}: {ms:.1f} ms/step -> {work / ms / 1e6:.2f} G res/s")
-        return ms, kern, n
+        r = bpx.scan_batch(ctx, b, ddb, launch=launch, phases=True).wait()
+        ph = phase_summary(r.phases())
+        r.close()
+        log(f"  {label}: {ms:.1f} ms/step -> {work / ms / 1e6:.2f} G res/s; phases {ph}")
+        return ms, kern, n, ph
 
-    ms_len_default, _, n0 = timed(None, "static longest-first order")
-    ms_model, kern, n2 = timed(cost_model, "learned-cost order")
+    ms_len_default, _, n0, ph_len = timed(None, "static longest-first order")
+    ms_model, kern, n2, ph_model = timed(cost_model, "learned-cost order")
     if n0 != n2:
         raise SystemExit(f"record counts differ between orders: {n0} {n2}")
     value = work / (ms_model / 1000.0)
-    bytes_alg = groups * (db.num_residues + 12 * db.num_subjects) + 12 * db.num_subjects * len(qs) + 24 * n2
+    bytes_alg = groups * (db.num_residues + 12 * db.num_subjects) + 24 * n2
     achieved = bytes_alg / (kern / 1000.0) / 1e9
     out = {"metric": METRIC, "value": round(value, 1), "unit": "DB residues/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_model, 3),
@@ -527,6 +562,7 @@ def skewed(args):
                       "group_queries": gq, "launch": launch,
                       "ms_longest_first": round(ms_len_default, 3),
                       "gain_over_longest_first": round(ms_len_default / ms_model, 4),
+                      "phases_longest_first": ph_len, "phases_learned": ph_model,
                       "cost_model_train_s": round(train_s, 2), "cost_model_inference_s": round(infer_s, 3)},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,

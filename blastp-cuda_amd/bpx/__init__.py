@@ -78,7 +78,7 @@ class PhaseStats(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("classes", C.c_uint32), ("span_ms", C.c_double),
                 ("wave_ms_mean", C.c_double), ("wave_ms_min", C.c_double), ("stage_ms_mean", C.c_double),
                 ("group_ms_mean", C.c_double), ("subjects_staged", C.c_uint64), ("bytes_staged", C.c_uint64),
-                ("flush_ms_mean", C.c_double), ("extensions", C.c_uint64)]
+                ("flush_ms_mean", C.c_double), ("extensions", C.c_uint64), ("busy_fraction", C.c_double)]
 
 
 class UngappedStats(C.Structure):

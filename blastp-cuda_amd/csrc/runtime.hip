@@ -1938,7 +1938,7 @@ int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
     bpx_phase_stats st{};
     st.waves = (uint32_t)nw;
     st.classes = (uint32_t)r->classes.size();
-    double life = 0, stage = 0, group = 0, flush = 0, life_min = 0;
+    double life = 0, stage = 0, group = 0, flush = 0, life_min = 0, slots = 0;
     uint64_t w0 = 0;
     for (const auto& c : r->classes) {
         const uint64_t cw = (uint64_t)r->ctx->num_cus * r->launch.groups_per_cu * c.waves;
@@ -1957,9 +1957,13 @@ int bpx_result_phases(const bpx_result* r, bpx_phase_stats* out) {
             flush += (double)p[6] * kTickMs;
             st.extensions += p[7];
         }
-        if (cw) st.span_ms += (double)(last - first) * kTickMs;
+        if (cw) {
+            st.span_ms += (double)(last - first) * kTickMs;
+            slots += (double)cw * (double)(last - first) * kTickMs;
+        }
         w0 += cw;
     }
+    st.busy_fraction = slots > 0 ? life / slots : 0.0;
     if (nw) {
         st.wave_ms_mean = life / (double)nw;
         st.wave_ms_min = life_min;

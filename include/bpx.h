@@ -252,7 +252,11 @@ int bpx_result_replay(bpx_result *r);
  * extension flushes (the queued two-hit X-drop extensions and the writing
  * of their records); extensions counts the extensions those flushes ran.
  * wave_ms_* are wavefront lifetimes; span_ms runs from the first start to
- * the last end of each launch class, summed. */
+ * the last end of each launch class, summed.  busy_fraction is the summed
+ * wavefront lifetimes over the wavefront slots of the launch classes
+ * (wavefronts x span of each): 1 - busy_fraction is the share of the
+ * launches' time that finished wavefronts sat idle waiting for the slowest,
+ * which bounds what any reordering of the work could save. */
 typedef struct bpx_phase_stats {
     uint32_t waves;
     uint32_t classes;
@@ -265,6 +269,7 @@ typedef struct bpx_phase_stats {
     uint64_t bytes_staged;
     double flush_ms_mean;
     uint64_t extensions;
+    double busy_fraction;
 } bpx_phase_stats;
 int bpx_result_phases(const bpx_result *r, bpx_phase_stats *out);
 int bpx_result_destroy(bpx_result *r);

@@ -613,6 +613,10 @@ def test_phase_times_are_consistent_and_change_nothing(ctx):
     # every extension the scan ran went through a flush
     assert ph["extensions"] == r.summary()[1]["init_extends"] > 0
     assert 0 < ph["flush_ms_mean"] < ph["wave_ms_mean"]
+    # one launch class: the busy fraction is the mean lifetime over the span
+    assert ph["classes"] == 1
+    assert abs(ph["busy_fraction"] - ph["wave_ms_mean"] / ph["span_ms"]) < 1e-6
+    assert 0 < ph["busy_fraction"] <= 1.0 + 1e-9
     r.close()
     dq.close()
     ddb.close()
