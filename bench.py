@@ -518,7 +518,11 @@ def skewed(args):
     train_s = time.perf_counter() - t
     log(f"  cost model: trained in {train_s:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}")
 
-    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
+    if args.policy:
+        gr, gq, qc, sb, cps = (int(x) for x in args.policy.split(","))
+        launch = {"queue_capacity": qc, "subject_buffer": sb, "claims_per_share": cps}
+    else:
+        gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
     b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
     groups = b.info()["groups"]
     log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch}; {groups} groups")

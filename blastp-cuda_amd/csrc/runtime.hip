@@ -1432,6 +1432,17 @@ int bpx_db_set_cost(bpx_db* db, const float* cost) {
             }
         }
     }
+    // Within a bin, subjects go longest first: its subjects cost about the
+    // same, and neighbours of similar length share a wavefront's subject
+    // buffer (a pack) as in the length order.
+    for (size_t b = 0; b < bin_end.size(); ++b) {
+        const uint32_t f = b ? bin_end[b - 1] : 0u;
+        std::stable_sort(order.begin() + f, order.begin() + bin_end[b], [&](uint32_t x, uint32_t y) {
+            const bool lx = db->h_len[x] >= kCostLongPrefix, ly = db->h_len[y] >= kCostLongPrefix;
+            if (lx != ly) return lx;
+            return db->h_len[x] > db->h_len[y];
+        });
+    }
     db->h_order.swap(order);
     HIP_TRY(upload_order(db));
     db->cost_order = cost != nullptr;
