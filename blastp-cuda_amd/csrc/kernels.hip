@@ -91,13 +91,18 @@ __device__ __forceinline__ uint32_t rank_below(unsigned long long mask) {
 // 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's
 // into row 3 (row_bcast:15), then lane 31's total into rows 2-3
 // (row_bcast:31).
+// bound_ctrl reads 0 for a lane whose source is outside the row, so each
+// step folds into one v_add_u32 with a DPP source operand.  The empty asm
+// hides the partial sums from the caller's arithmetic: otherwise the
+// compiler rebuilds incl - v from them and keeps the separate moves.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, true);
+    asm volatile("" : "+v"(v));
     return v;
 }
 
@@ -842,7 +847,9 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 const uint32_t h2 = (__umul24(dslot, 0x9E3779u) >> 16) & cmask;  // 24-bit multiply: full rate
                 bool deferred = false;
                 const int32_t sd = s + doff;
-                for (int round = 0; (kRounds == 0 || round < kRounds) && __ballot(active) != 0; ++round) {
+                // (n >= 1: round 0 always has an active lane)
+                for (int round = 0; (kRounds == 0 || round < kRounds) && (round == 0 || __ballot(active) != 0);
+                     ++round) {
                     if (round > 0) h = h2;
                     if (active) atomicMin(&claim[h], lane);
                     wave_sync();
