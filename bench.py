@@ -92,6 +92,33 @@ def measured_traffic(name, policy=None):
     return t.get("bytes_per_launch"), f"profiles/{name}.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
 
 
+# Vector-issue peak: 256 CUs x 4 SIMDs, one wave64 vector instruction per
+# SIMD every 4 cycles (16 lanes wide), at the 2.4 GHz peak engine clock.
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def issue_roofline(name, kernel_ms, policy=None):
+    """The vector-issue roofline of the scan kernel: the vector instructions
+    per launch from a committed rocprofv3 SQ_INSTS_VALU summary
+    (profiles/<name>.json, used only while the kernel source is the one it
+    was measured on) over the kernel time of this run, against the peak
+    issue rate of the 1,024 SIMDs."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    out = {"bound": "vector issue", "unit": "G wave64 VALU instructions/s", "peak": VALU_PEAK_G}
+    if not os.path.exists(path):
+        return dict(out, achieved=None, frac=None, source="not measured")
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernels_hip_sha256") != kernel_source_sha():
+        return dict(out, achieved=None, frac=None, source=f"profiles/{name}.json was measured on another kernel source")
+    if policy is not None and t.get("policy") != policy:
+        return dict(out, achieved=None, frac=None,
+                    source=f"profiles/{name}.json was measured under packing policy {t.get('policy')}, not {policy}")
+    achieved = t["valu_per_launch"] / (kernel_ms / 1e3) / 1e9
+    return dict(out, achieved=round(achieved, 1), frac=round(achieved / VALU_PEAK_G, 4),
+                valu_per_launch=t["valu_per_launch"], source=f"profiles/{name}.json (rocprofv3 --pmc SQ_INSTS_VALU)")
+
+
 def phase_summary(ph):
     """Per-wavefront phase times of one scan (bpx_result_phases, measured
     outside the timed steps).  The subject-streaming phase -- descriptor
@@ -272,7 +299,9 @@ def plain(args):
                       "init_extends": P["ust"]["init_extends"], "db_load_s": round(P["load_s"], 4),
                       "cold_search_residues_per_s": round(db.num_residues / (P["load_s"] + P["ms"] / 1000.0), 1),
                       "phases": P["phases"]},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline,
+           "roofline_issue": issue_roofline("sq_counters_k_scan_plain_500aa_2M_subjects", P["kern"]),
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -406,7 +435,8 @@ def headline(args):
               "roofline_frac": round(P["bytes_alg"] / (P["kern"] / 1000.0) / 1e9 / HBM_PEAK_GBS, 5),
               "db_load_s": round(P["load_s"], 4),
               "cold_search_residues_per_s": round(P["db"].num_residues / (P["load_s"] + P["ms"] / 1000.0), 1),
-              "phases": P["phases"]}
+              "phases": P["phases"],
+              "roofline_issue": issue_roofline("sq_counters_k_scan_plain_500aa_2M_subjects", P["kern"])}
     P["dq"].close()
     P["ddb"].close()
 
@@ -444,7 +474,10 @@ def headline(args):
                       "gain_over_longest_first": round(ms_lf / ms, 4),
                       "cost_model_train_s": round(train_s, 2), "cost_model_inference_s": round(infer_s, 3),
                       "phases": batched_phases, "plain_2M": plain2},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline,
+           "roofline_issue": issue_roofline("sq_counters_k_scan_batched_1024_queries_500k_subjects", kern,
+                                            {"group_residues": gr, "group_queries": gq}),
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
