@@ -75,6 +75,19 @@ def kernel_source_sha():
         return hashlib.sha256(f.read()).hexdigest()
 
 
+def same_policy(a, b):
+    """Packing policies compared field by field (a missing field is 0)."""
+    keys = ("group_residues", "group_queries", "group_diag_slots")
+    return a is not None and all(int(a.get(k, 0)) == int(b.get(k, 0)) for k in keys)
+
+
+def parse_policy(text):
+    """--policy GR,GQ,QCAP,SBUF,CLAIMS[,DIAG] -> (policy, launch)."""
+    f = [int(x) for x in text.split(",")]
+    policy = {"group_residues": f[0], "group_queries": f[1], "group_diag_slots": f[5] if len(f) > 5 else 0}
+    return policy, {"queue_capacity": f[2], "subject_buffer": f[3], "claims_per_share": f[4]}
+
+
 def measured_traffic(name, policy=None):
     """HBM bytes per step from a committed rocprofv3 --pmc summary
     (profiles/<name>.json), or (None, reason).  A batched figure holds for
@@ -87,7 +100,7 @@ def measured_traffic(name, policy=None):
         t = json.load(f)
     if t.get("kernels_hip_sha256") != kernel_source_sha():
         return None, f"profiles/{name}.json was measured on another kernel source"
-    if policy is not None and t.get("policy") != policy:
+    if policy is not None and not same_policy(t.get("policy"), policy):
         return None, f"profiles/{name}.json was measured under packing policy {t.get('policy')}, not {policy}"
     return t.get("bytes_per_launch"), f"profiles/{name}.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
 
@@ -111,7 +124,7 @@ def issue_roofline(name, kernel_ms, policy=None):
         t = json.load(f)
     if t.get("kernels_hip_sha256") != kernel_source_sha():
         return dict(out, achieved=None, frac=None, source=f"profiles/{name}.json was measured on another kernel source")
-    if policy is not None and t.get("policy") != policy:
+    if policy is not None and not same_policy(t.get("policy"), policy):
         return dict(out, achieved=None, frac=None,
                     source=f"profiles/{name}.json was measured under packing policy {t.get('policy')}, not {policy}")
     achieved = t["valu_per_launch"] / (kernel_ms / 1e3) / 1e9
@@ -336,14 +349,13 @@ def headline(args):
     # or given with --policy
     t = time.perf_counter()
     if args.policy:
-        gr, gq, qc, sb, cps = (int(x) for x in args.policy.split(","))
-        launch = {"queue_capacity": qc, "subject_buffer": sb, "claims_per_share": cps}
-        log(f"  policy given: group_residues={gr} group_queries={gq} launch={launch}")
+        policy, launch = parse_policy(args.policy)
+        log(f"  policy given: {policy} launch={launch}")
     else:
-        gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
-        log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch} ({time.perf_counter() - t:.1f} s, "
+        policy, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
+        log(f"  autotune: {policy} launch={launch} ({time.perf_counter() - t:.1f} s, "
             f"{len(trials)} trials, each printed above as it finished)")
-    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    b = ctx.prepare_batch(qs, **policy)
     groups = b.info()["groups"]
     t = time.perf_counter()
     X = bpx.db_features(ctx, ddb, counts)
@@ -447,7 +459,7 @@ def headline(args):
     bytes_alg = groups * (db.num_residues + 12 * db.num_subjects) + 24 * nrec
     achieved = bytes_alg / (kern / 1000.0) / 1e9
     traffic, tsrc = measured_traffic("traffic_k_scan_batched_1024_queries_500k_subjects",
-                                   {"group_residues": gr, "group_queries": gq})
+                                   policy)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": "k_scan (all launch classes of one batched step)",
@@ -466,7 +478,7 @@ def headline(args):
                                   "500,000-subject DB",
                       "queries": nq, "query_residues": qres, "subjects": db.num_subjects,
                       "residues": db.num_residues, "db_sha256": db.sha256, "records": int(nrec),
-                      "groups": groups, "group_residues": gr, "group_queries": gq, "launch": launch,
+                      "groups": groups, "policy": policy, "launch": launch,
                       "plain_value": round(plain_value, 1),
                       "plain_seconds": round(float(np.median(plain_times)), 3),
                       "batched_over_plain": round(value / plain_value, 3),
@@ -476,7 +488,7 @@ def headline(args):
                       "phases": batched_phases, "plain_2M": plain2},
            "roofline": roofline,
            "roofline_issue": issue_roofline("sq_counters_k_scan_batched_1024_queries_500k_subjects", kern,
-                                            {"group_residues": gr, "group_queries": gq}),
+                                            policy),
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -492,9 +504,9 @@ def main():
     ap.add_argument("--workload", choices=("headline", "batched", "plain", "skewed", "serving"), default="headline")
     ap.add_argument("--rates", default="25,50,200,2000", help="serving: arrival rates (queries/s), comma-separated")
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per arrival rate")
-    ap.add_argument("--policy", default="", metavar="GR,GQ,QCAP,SBUF,CLAIMS",
-                    help="headline: group residues, group queries, hit-ring capacity, subject buffer and claims "
-                         "per share to use instead of autotuning")
+    ap.add_argument("--policy", default="", metavar="GR,GQ,QCAP,SBUF,CLAIMS[,DIAG]",
+                    help="headline and skewed: group residues, group queries, hit-ring capacity, subject buffer, "
+                         "claims per share and (optional) group diagonal slots to use instead of autotuning")
     ap.add_argument("--only-batched", action="store_true",
                     help="headline: time the batched graph only (no longest-first, per-query or 2,000,000-subject "
                          "legs), for profiling runs; prints a reduced line")
@@ -552,13 +564,12 @@ def skewed(args):
     log(f"  cost model: trained in {train_s:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}")
 
     if args.policy:
-        gr, gq, qc, sb, cps = (int(x) for x in args.policy.split(","))
-        launch = {"queue_capacity": qc, "subject_buffer": sb, "claims_per_share": cps}
+        policy, launch = parse_policy(args.policy)
     else:
-        gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
-    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+        policy, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects)
+    b = ctx.prepare_batch(qs, **policy)
     groups = b.info()["groups"]
-    log(f"  autotune: group_residues={gr} group_queries={gq} launch={launch}; {groups} groups")
+    log(f"  autotune: {policy} launch={launch}; {groups} groups")
     t = time.perf_counter()
     X = bpx.db_features(ctx, ddb, counts)
     cost_model = costsched.predicted_cost(model, X, db.lengths, len(qs), len(sample), groups)
@@ -595,8 +606,8 @@ def skewed(args):
            "data": "synthetic (seeded generator: skewed lengths, 20 % biased composition with tandem repeats)",
            "config": {"workload": "batched + learned-cost order: 1,024 queries vs skewed 500,000-subject DB",
                       "queries": len(qs), "subjects": db.num_subjects, "residues": db.num_residues,
-                      "db_sha256": db.sha256, "records": int(n2), "groups": groups, "group_residues": gr,
-                      "group_queries": gq, "launch": launch,
+                      "db_sha256": db.sha256, "records": int(n2), "groups": groups, "policy": policy,
+                      "launch": launch,
                       "ms_longest_first": round(ms_len_default, 3),
                       "gain_over_longest_first": round(ms_len_default / ms_model, 4),
                       "phases_longest_first": ph_len, "phases_learned": ph_model,

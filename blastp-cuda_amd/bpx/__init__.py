@@ -57,7 +57,7 @@ class Launch(C.Structure):
 
 
 class BatchPolicy(C.Structure):
-    _fields_ = [("group_residues", C.c_int32), ("group_queries", C.c_int32)]
+    _fields_ = [("group_residues", C.c_int32), ("group_queries", C.c_int32), ("group_diag_slots", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -352,8 +352,9 @@ class Context:
         return Query(self, query, params, matrix)
 
     def prepare_batch(self, queries, params: WordParams | None = None, matrix=None, x_dropoff=None,
-                      cutoff=None, group_residues=0, group_queries=0) -> "Batch":
-        return Batch(self, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries)
+                      cutoff=None, group_residues=0, group_queries=0, group_diag_slots=0) -> "Batch":
+        return Batch(self, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries,
+                     group_diag_slots)
 
 
 class Database:
@@ -413,7 +414,8 @@ class Query:
 class Batch:
     """Many queries staged together (packed into scan groups)."""
 
-    def __init__(self, ctx: Context, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries):
+    def __init__(self, ctx: Context, queries, params, matrix, x_dropoff, cutoff, group_residues, group_queries,
+                 group_diag_slots=0):
         self.ctx = ctx
         self.seqs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
         self.num_queries = len(self.seqs)
@@ -423,7 +425,7 @@ class Batch:
         m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.int8)
         xd = None if x_dropoff is None else np.ascontiguousarray(x_dropoff, dtype=np.int32)
         co = None if cutoff is None else np.ascontiguousarray(cutoff, dtype=np.int32)
-        pol = BatchPolicy(group_residues, group_queries)
+        pol = BatchPolicy(group_residues, group_queries, group_diag_slots)
         self._h = _vp()
         _check(_lib.bpx_batch_prepare(ctx._h, self.num_queries, ptrs, _ptr(lens), _ptr(m), C.byref(self.params),
                                       _ptr(xd), _ptr(co), C.byref(pol), C.byref(self._h)))
@@ -599,7 +601,8 @@ def autotune_batch(ctx: Context, queries, db: Database, sample_first: int = 0, s
                    params: WordParams | None = None, matrix=None, x_dropoff=None, cutoff=None):
     """Times candidate packing policies / launch shapes on subjects
     [sample_first, sample_first + sample_count) and returns
-    (group_residues, group_queries, launch dict, trials)."""
+    (policy dict -- group_residues, group_queries, group_diag_slots, the
+    keyword arguments of Context.prepare_batch -- launch dict, trials)."""
     seqs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
     if sample_count is None:
         sample_count = max(1, db.num_subjects // 20)
@@ -614,9 +617,9 @@ def autotune_batch(ctx: Context, queries, db: Database, sample_first: int = 0, s
     nt = C.c_uint32()
     _check(_lib.bpx_autotune_batch(ctx._h, len(seqs), ptrs, _ptr(lens), _ptr(m), C.byref(prm), _ptr(xd), _ptr(co),
                                    db._h, sample_first, sample_count, C.byref(out), trials, 64, C.byref(nt)))
-    tr = [{"group_residues": t.policy.group_residues, "group_queries": t.policy.group_queries,
-           "launch": _struct_dict(t.launch), "sample_ms": round(float(t.sample_ms), 3)} for t in trials[:min(nt.value, 64)]]
-    return out.policy.group_residues, out.policy.group_queries, _struct_dict(out.launch), tr
+    tr = [{"policy": _struct_dict(t.policy), "launch": _struct_dict(t.launch), "sample_ms": round(float(t.sample_ms), 3)}
+          for t in trials[:min(nt.value, 64)]]
+    return _struct_dict(out.policy), _struct_dict(out.launch), tr
 
 
 CM_FEATURES = 32
@@ -713,13 +716,14 @@ class Server:
     fetches its records.  Each result equals scanning the query alone."""
 
     def __init__(self, ctx: Context, db: Database, max_batch=0, max_inflight=0, max_wait_us=0, hbm_budget=0,
-                 group_residues=0, group_queries=0, launch=None, params: WordParams | None = None, diag_budget=0):
+                 group_residues=0, group_queries=0, launch=None, params: WordParams | None = None, diag_budget=0,
+                 group_diag_slots=0):
         cfg = ServerConfig()
         cfg.max_batch, cfg.max_inflight, cfg.max_wait_us, cfg.hbm_budget = max_batch, max_inflight, max_wait_us, hbm_budget
         cfg.diag_budget = diag_budget
         if params is not None:
             cfg.params = params
-        cfg.policy = BatchPolicy(group_residues, group_queries)
+        cfg.policy = BatchPolicy(group_residues, group_queries, group_diag_slots)
         if launch is not None:
             cfg.launch = _launch(launch)
         self.ctx, self.db = ctx, db

@@ -5,11 +5,14 @@
 // the library measures with HIP events.
 //
 // Search (coordinate descent, each step keeps the best so far):
-//   1. packing policy: group_residues, then group_queries -- how many
-//      queries share one pass over a subject, hence the per-wave
-//      diagonal-table size (the sum of the members' diagonal lengths; the
-//      query cap bounds it for queries of one length class) and the
-//      lookup-table placement.  A group pass has fixed costs (restaging,
+//   1. packing policy: group_residues, then group_queries, then
+//      group_diag_slots -- how many queries share one pass over a subject,
+//      hence the per-wave diagonal-table size (the sum of the members'
+//      diagonal lengths) and the lookup-table placement.  The query cap bounds
+//      the diagonal table only for queries of one length class; the slot cap
+//      bounds it for every group, so a group of longer queries still fits 16
+//      wavefronts per CU (1,024-query batch: packed groups 1,087 -> 1,034 ms
+//      with 3,072 slots and a 256-entry ring).  A group pass has fixed costs (restaging,
 //      the slowest wavefront at the group change) that weigh more on a small
 //      sample: on the 1,024-query workload a tenth of the database picked
 //      groups 4 % slower at full size than the best, a quarter could not
@@ -92,10 +95,11 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         bpx_batch_destroy(b);
         if (r == BPX_OK) record(c, *ms);
         if (log)
-            std::fprintf(stderr, "bpx autotune: residues %u queries %u ring %u buffer %u claims %u: %s %.2f ms\n",
-                         c.policy.group_residues, c.policy.group_queries, c.launch.queue_capacity,
-                         c.launch.subject_buffer, c.launch.claims_per_share, r == BPX_OK ? "ok" : "refused",
-                         r == BPX_OK ? *ms : 0.f);
+            std::fprintf(stderr,
+                         "bpx autotune: residues %d queries %d diag slots %d ring %d buffer %d claims %d: %s %.2f ms\n",
+                         c.policy.group_residues, c.policy.group_queries, c.policy.group_diag_slots,
+                         c.launch.queue_capacity, c.launch.subject_buffer, c.launch.claims_per_share,
+                         r == BPX_OK ? "ok" : "refused", r == BPX_OK ? *ms : 0.f);
         return r;
     };
 
@@ -137,6 +141,17 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
         Candidate c = base0;
         c.policy.group_queries = gq;
         if ((rc = consider(c)) != BPX_OK) return rc;
+    }
+    // the diagonal-table bound, with the chosen query cap and with 16
+    const Candidate base1 = best;
+    for (int32_t ds : {2048, 3072}) {
+        for (int32_t gq : {0, 16}) {
+            Candidate c = base1;
+            c.policy.group_diag_slots = ds;
+            if (gq) c.policy.group_queries = std::max(gq, base1.policy.group_queries);
+            if (gq && c.policy.group_queries == base1.policy.group_queries) continue;
+            if ((rc = consider(c)) != BPX_OK) return rc;
+        }
     }
     const Candidate base = best;
     for (int32_t sb : {2048, 4096}) {

@@ -273,7 +273,7 @@ struct QuerySpec {
 };
 
 int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, int32_t word, int32_t threshold,
-              int32_t window, int32_t group_residues, int32_t group_queries, QuerySet* out,
+              int32_t window, int32_t group_residues, int32_t group_queries, int32_t group_diag_slots, QuerySet* out,
               std::vector<bpx::HostTable>* keep_tables = nullptr) {
     QuerySet& set = *out;
     set.ctx = ctx;
@@ -288,10 +288,12 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
     std::vector<std::vector<uint32_t>> members;
     for (size_t k = 0; k < order.size();) {
         std::vector<uint32_t> m{order[k]};
-        int64_t tot = qs[order[k]].len;
+        int64_t tot = qs[order[k]].len, dsum = qs[order[k]].diag_len;
         ++k;
         if (tot < group_residues)
-            while (k < order.size() && (int32_t)m.size() < group_queries && tot + qs[order[k]].len <= group_residues) {
+            while (k < order.size() && (int32_t)m.size() < group_queries && tot + qs[order[k]].len <= group_residues &&
+                   (group_diag_slots <= 0 || dsum + qs[order[k]].diag_len <= group_diag_slots)) {
+                dsum += qs[order[k]].diag_len;
                 tot += qs[order[k]].len;
                 m.push_back(order[k]);
                 ++k;
@@ -1519,7 +1521,7 @@ int bpx_query_prepare(bpx_ctx* ctx, const uint8_t* query, int32_t qlen, const in
     q->params = p;
     std::vector<QuerySpec> qs{{query, qlen, p.x_dropoff, p.cutoff_score, p.diag_mask + 1}};
     std::vector<bpx::HostTable> tabs;
-    int rc = build_set(ctx, qs, M, p.word_length, p.threshold, p.window, 0, 1, &q->set, &tabs);
+    int rc = build_set(ctx, qs, M, p.word_length, p.threshold, p.window, 0, 1, 0, &q->set, &tabs);
     if (rc) {
         delete q;
         return rc;
@@ -1579,7 +1581,8 @@ int bpx_batch_prepare(bpx_ctx* ctx, uint32_t num_queries, const uint8_t* const* 
     gres = std::min(gres, 65535);
     auto* b = new (std::nothrow) bpx_batch;
     if (!b) return fail(BPX_E_NOMEM, "host allocation failed");
-    int rc = build_set(ctx, qs, matrix ? matrix : bpx::blosum62(), p.word_length, p.threshold, p.window, gres, gq,
+    const int32_t gd = policy && policy->group_diag_slots > 0 ? policy->group_diag_slots : 0;
+    int rc = build_set(ctx, qs, matrix ? matrix : bpx::blosum62(), p.word_length, p.threshold, p.window, gres, gq, gd,
                        &b->set);
     if (rc) {
         delete b;

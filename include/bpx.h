@@ -115,12 +115,17 @@ typedef struct bpx_launch {
 } bpx_launch;
 
 /* How a batch packs its queries into scan groups: queries shorter than
- * group_residues are concatenated (at most group_queries per group) and share
- * one lookup table and one pass over every subject; longer queries form
- * groups of one.  Results do not depend on the packing. */
+ * group_residues are concatenated (at most group_queries per group, and while
+ * their diagonal-table lengths sum to at most group_diag_slots) and share one
+ * lookup table and one pass over every subject; longer queries form groups
+ * of one.  A group's summed diagonal-table length is the diagonal table each
+ * wavefront holds in LDS while it scans the group, so group_diag_slots bounds
+ * that table and with it how many wavefronts fit a CU.  Results do not
+ * depend on the packing. */
 typedef struct bpx_batch_policy {
     int32_t group_residues;   /* 0 = default */
     int32_t group_queries;    /* 0 = default, at most 255 */
+    int32_t group_diag_slots; /* 0 = no bound beyond the two above */
 } bpx_batch_policy;
 
 /* Output selection for a scan. */
@@ -308,10 +313,11 @@ int bpx_scan_noalloc(bpx_ctx *ctx, const bpx_query *q, const bpx_db *db, bpx_ini
  * fastest.  No reference counterpart: the reference takes its launch shape
  * from -gpu_blocks / -gpu_threads (cmdline_flags.cpp:218-222).  Coordinate
  * search, each step keeping the best so far: packing policy -- residues per
- * group, then queries per group (which bound each group's per-wavefront LDS
- * diagonal table, the sum of its members' diagonal lengths, and set its
- * table placement) -- then subject buffer, hit-ring
- * capacity, then subjects per wavefront claim (claims_per_share), the last
+ * group, queries per group, then a bound on each group's summed diagonal
+ * slots (the per-wavefront LDS diagonal table; these set how many wavefronts
+ * fit a CU and the table placement) -- then subject buffer, hit-ring
+ * capacity, residues once more, then subjects per wavefront claim
+ * (claims_per_share), the last
  * timed on twice the sample (or the whole database) since the claim
  * size follows the number of subjects.  The arguments before db are those of
  * bpx_batch_prepare.  trials (nullable) receives every configuration timed.

@@ -462,7 +462,7 @@ def test_batch_per_query_parameters_and_policies(ctx):
     xd = rng.integers(8, 30, len(queries)).astype(np.int32)
     co = rng.integers(25, 50, len(queries)).astype(np.int32)
     for pol in (dict(group_residues=300, group_queries=4), dict(group_residues=4000, group_queries=200),
-                dict(group_residues=1)):
+                dict(group_residues=4000, group_queries=64, group_diag_slots=1536), dict(group_residues=1)):
         say(f"policy {pol}")
         assert_batch_parity(ctx, queries, db, xd, co, **pol)
 
@@ -490,11 +490,11 @@ def test_autotuned_batch_matches_oracle(ctx):
     queries = [bpx.synth_query(23, i, L) for i, L in enumerate(lens)]
     db = batch_db(queries, n=2000, seed=33)
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, queries, ddb, 100, 400)
-    say(f"autotune: {gr} {gq} {launch}; {len(trials)} trials")
+    policy, launch, trials = bpx.autotune_batch(ctx, queries, ddb, 100, 400)
+    say(f"autotune: {policy} {launch}; {len(trials)} trials")
     assert len(trials) >= 4 and all(t["sample_ms"] > 0 for t in trials)
     ddb.close()
-    assert_batch_parity(ctx, queries, db, launch=launch, group_residues=gr, group_queries=gq)
+    assert_batch_parity(ctx, queries, db, launch=launch, **policy)
 
 
 def test_graph_replay_equals_eager(ctx):

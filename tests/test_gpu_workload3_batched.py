@@ -30,10 +30,9 @@ def batched(ctx):
     qs, db = W.config3(THREADS)
     assert db.sha256 == W.SHA256[3]
     ddb = ctx.load_db(db.residues, db.offsets[:-1], db.lengths)
-    gr, gq, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
-    say(f"batched workload: autotuned group_residues={gr} group_queries={gq} launch={launch} "
-        f"({time.perf_counter() - t:.1f} s)")
-    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    policy, launch, trials = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    say(f"batched workload: autotuned {policy} launch={launch} ({time.perf_counter() - t:.1f} s)")
+    b = ctx.prepare_batch(qs, **policy)
     g = bpx.scan_batch(ctx, b, ddb, launch=launch, subject_stats=True, graph=True).wait()
     snap = snapshot(g, len(qs), db.num_subjects)
     say(f"batched workload: {len(snap['recs'])} records ({time.perf_counter() - t:.1f} s)")

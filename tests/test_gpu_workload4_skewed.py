@@ -34,8 +34,8 @@ def skewed(ctx):
     model, counts, losses = costsched.fit(ctx, W.CONFIG4_SPEC, sample, batches=100, subjects=2000, epochs=8)
     say(f"skewed workload: cost model trained, losses {[round(float(x), 4) for x in losses]} "
         f"({time.perf_counter() - t:.1f} s)")
-    gr, gq, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
-    b = ctx.prepare_batch(qs, group_residues=gr, group_queries=gq)
+    policy, launch, _ = bpx.autotune_batch(ctx, qs, ddb, 0, db.num_subjects // 10)
+    b = ctx.prepare_batch(qs, **policy)
     X = bpx.db_features(ctx, ddb, counts)
     cost = costsched.predicted_cost(model, X, db.lengths, len(qs), len(sample), b.info()["groups"])
     model.close()
