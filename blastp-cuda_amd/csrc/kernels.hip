@@ -556,6 +556,53 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
     uint32_t cur_group = 0xFFFFFFFFu;
     GroupInfo g{};
 
+    // per-query ungapped statistics (Blast_UngappedStatsUpdate), summed over
+    // the items this wave ran of one group -- a group's cost bins are
+    // consecutive items -- and added to the launch's totals when the group
+    // changes (flat) or the item ends (claimed items); lane j owns the
+    // group's queries j, j + 64, ...
+    uint32_t a_nseq = 0, a_ext = 0, a_saved = 0, a_passed = 0;  // single-query aggregates
+    unsigned long long a_hits = 0;
+    unsigned long long m_hits[kQuerySlots];                        // MULTI: per owned query
+    uint32_t m_nseq[kQuerySlots], m_ext[kQuerySlots], m_saved[kQuerySlots], m_passed[kQuerySlots];
+    auto zero_ustats = [&]() {
+        a_nseq = a_ext = a_saved = a_passed = 0;
+        a_hits = 0;
+#pragma unroll
+        for (int t = 0; t < kQuerySlots; ++t) {
+            m_hits[t] = 0;
+            m_nseq[t] = m_ext[t] = m_saved[t] = m_passed[t] = 0;
+        }
+    };
+    // reads the current group's qinfo: called before the group changes
+    auto flush_ustats = [&]() {
+        if (!MULTI) {
+            if (lane == 0 && a_nseq) {
+                unsigned long long* u = p.ustats + 5 * (size_t)qinfo[0].qid;
+                atomicAdd(&u[0], a_hits);
+                atomicAdd(&u[1], (unsigned long long)a_nseq);
+                atomicAdd(&u[2], (unsigned long long)a_ext);
+                atomicAdd(&u[3], (unsigned long long)a_saved);
+                atomicAdd(&u[4], (unsigned long long)a_passed);
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < kQuerySlots; ++t) {
+                const uint32_t jj = lane + 64u * (uint32_t)t;
+                if (jj < g.nq && m_nseq[t]) {
+                    unsigned long long* u = p.ustats + 5 * (size_t)qinfo[jj].qid;
+                    atomicAdd(&u[0], m_hits[t]);
+                    atomicAdd(&u[1], (unsigned long long)m_nseq[t]);
+                    atomicAdd(&u[2], (unsigned long long)m_ext[t]);
+                    atomicAdd(&u[3], (unsigned long long)m_saved[t]);
+                    atomicAdd(&u[4], (unsigned long long)m_passed[t]);
+                }
+            }
+        }
+        zero_ustats();
+    };
+    zero_ustats();
+
     uint32_t pass = 0;               // flat: items in order, then done
     uint32_t c_next = 0, c_end = 0;  // flat: this wave's claimed subjects (wave-uniform)
     for (;;) {
@@ -578,6 +625,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
         }
         ++pass;
         c_next = c_end = 0;
+        // flat: this wave is done with its group when the next item belongs
+        // to another (each wave reads qinfo before the barrier of the change)
+        if (p.flat && cur_group != 0xFFFFFFFFu && (item >= p.nitems || p.items[item].group != cur_group))
+            flush_ustats();
         if (item >= p.nitems) {
             // largest record count any wave produced: sizes a rescan after
             // a segment overflow exactly
@@ -623,15 +674,6 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
         const uint16_t* entries = narrow_starts ? starts16 + ((g.npresent + 2u) & ~1u)
                                                 : reinterpret_cast<const uint16_t*>(starts32 + g.npresent + 1);
         const QueryInfo q0 = qinfo[0];
-        uint32_t a_nseq = 0, a_ext = 0, a_saved = 0, a_passed = 0;  // single-query aggregates
-        unsigned long long m_hits[kQuerySlots];                        // MULTI: per owned query
-        uint32_t m_nseq[kQuerySlots], m_ext[kQuerySlots], m_saved[kQuerySlots], m_passed[kQuerySlots];
-#pragma unroll
-        for (int t = 0; t < kQuerySlots; ++t) {
-            m_hits[t] = 0;
-            m_nseq[t] = m_ext[t] = m_saved[t] = m_passed[t] = 0;
-        }
-        unsigned long long a_hits = 0;
         // flat: the descriptors of the claimed chunk, lane j holding those of
         // its j-th subject -- loaded with the claim, so a subject's residue
         // loads wait for one memory round trip, not two
@@ -1313,30 +1355,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             }
         }
 
-        // ---- per-query ungapped statistics of this work item ----
-        if (!MULTI) {
-            if (lane == 0 && a_nseq) {
-                unsigned long long* u = p.ustats + 5 * (size_t)q0.qid;
-                atomicAdd(&u[0], a_hits);
-                atomicAdd(&u[1], (unsigned long long)a_nseq);
-                atomicAdd(&u[2], (unsigned long long)a_ext);
-                atomicAdd(&u[3], (unsigned long long)a_saved);
-                atomicAdd(&u[4], (unsigned long long)a_passed);
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < kQuerySlots; ++t) {
-                const uint32_t jj = lane + 64u * (uint32_t)t;
-                if (jj < g.nq && m_nseq[t]) {
-                    unsigned long long* u = p.ustats + 5 * (size_t)qinfo[jj].qid;
-                    atomicAdd(&u[0], m_hits[t]);
-                    atomicAdd(&u[1], (unsigned long long)m_nseq[t]);
-                    atomicAdd(&u[2], (unsigned long long)m_ext[t]);
-                    atomicAdd(&u[3], (unsigned long long)m_saved[t]);
-                    atomicAdd(&u[4], (unsigned long long)m_passed[t]);
-                }
-            }
-        }
+        // claimed items: the next item may belong to another group
+        if (!p.flat) flush_ustats();
     }
 }
 
