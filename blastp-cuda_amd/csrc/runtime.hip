@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -62,6 +63,9 @@ struct Pool {
     explicit Pool(unsigned host_flags) : host_(true), flags_(host_flags) {}
     bool host_ = false;
     unsigned flags_ = 0;
+    // tests: this many allocations first make a request HIP must refuse (a
+    // real hipErrorOutOfMemory), so the release-and-retry path below runs
+    int fail_first = 0;
     std::multimap<size_t, void*> free_;
     std::map<void*, size_t> sizes_;
     ~Pool() {
@@ -85,13 +89,27 @@ struct Pool {
             return p;
         }
         void* p = nullptr;
-        if (alloc(&p, bytes) != hipSuccess) {
+        hipError_t e;
+        if (fail_first > 0) {
+            --fail_first;
+            e = alloc(&p, size_t(1) << 62);
+        } else {
+            e = alloc(&p, bytes);
+        }
+        if (e != hipSuccess) {
+            // the failure is handled here (the cache is released and the
+            // allocation retried): clear it from the thread's last error,
+            // which every kernel launch check reads
+            (void)hipGetLastError();
             for (auto& kv : free_) {
                 release(kv.second);
                 sizes_.erase(kv.second);
             }
             free_.clear();
-            if (alloc(&p, bytes) != hipSuccess) return nullptr;
+            if (alloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
         }
         sizes_[p] = bytes;
         return p;
@@ -1302,9 +1320,17 @@ int bpx_context_create(int device, bpx_ctx** out) {
     // the allocation-free scan's scratch on this device (current); a failure
     // is reported by bpx_scan_noalloc, the only user
     e = scratch_address(device, &ctx->scratch);
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
         std::snprintf(ctx->scratch_err, sizeof ctx->scratch_err, "hipGetSymbolAddress(scratch) on device %d: %s",
                       device, hipGetErrorString(e));
+        (void)hipGetLastError();  // kept for bpx_scan_noalloc, not for the next launch check
+    }
+    // BPX_POOL_FAIL_FIRST=n (tests): the first n device-pool allocations of
+    // this context fail once before their retry
+    if (const char* f = std::getenv("BPX_POOL_FAIL_FIRST")) ctx->pool.fail_first = std::max(0, std::atoi(f));
+    // the attribute and memory queries above fall back to defaults on
+    // failure: none of their errors may reach a later launch check
+    (void)hipGetLastError();
     *out = ctx;
     return BPX_OK;
 }

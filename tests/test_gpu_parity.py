@@ -349,6 +349,28 @@ def test_device_memory_limit_refuses_a_scan_that_would_not_fit(ctx):
     db.close()
 
 
+def test_refused_pool_allocation_is_retried_without_a_stale_error(monkeypatch):
+    """The device pool retries an allocation HIP refuses after releasing its
+    cached blocks.  The refusal is handled there, so it must not reach the
+    next kernel launch check, which reads the thread's last HIP error (a
+    stale out-of-memory once failed an autotuner trial that way).  With
+    BPX_POOL_FAIL_FIRST set, every device-pool allocation of a new context
+    first asks HIP for 4 EiB: a plain and a batched scan run through that
+    path and still equal the oracle."""
+    monkeypatch.setenv("BPX_POOL_FAIL_FIRST", "100000")
+    c = bpx.Context(0)
+    try:
+        q = bpx.synth_query(6, 1, 250)
+        spec = dict(seed=9, num_subjects=1500, len_median=300, len_sigma=0.6, len_min=30, len_max=3000)
+        sdb = bpx.synth_db(spec, [q], nthreads=THREADS)
+        res, offs, lens = sdb.residues, sdb.offsets, sdb.lengths
+        assert_parity(gpu_run(c, q, res, offs, lens), oracle_run(q, res, offs, lens))
+        queries = [bpx.synth_query(26, i, L) for i, L in enumerate([40, 120, 300, 700, 1500])]
+        assert_batch_parity(c, queries, batch_db(queries, n=800, seed=36), group_residues=1000, group_queries=4)
+    finally:
+        c.close()
+
+
 def test_query_footprint(ctx):
     """The footprint of a query scanned alone: its diagonal slots (next power
     of two >= qlen + window), the waves it runs at with its table in HBM; a
