@@ -130,6 +130,7 @@ _sig("bpx_query_info", C.c_int, _vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
      C.POINTER(C.c_int64))
 _sig("bpx_query_destroy", C.c_int, _vp)
 _sig("bpx_context_limit", C.c_int, _vp, C.c_int64)
+_sig("bpx_context_trim", C.c_int, _vp)
 _sig("bpx_query_footprint", C.c_int, _vp, _vp, C.c_int32, C.POINTER(WordParams), C.POINTER(Launch),
      C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_size_t))
 _sig("bpx_check_memory", C.c_int, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(Launch),
@@ -333,6 +334,10 @@ class Context:
     def limit(self, hbm_bytes: int):
         """bpx_context_limit: cap the device bytes scans are planned with (0: the device's)."""
         _check(_lib.bpx_context_limit(self._h, hbm_bytes))
+
+    def trim(self):
+        """bpx_context_trim: free the buffers the context keeps cached for later scans."""
+        _check(_lib.bpx_context_trim(self._h))
 
     def close(self):
         if self._h:

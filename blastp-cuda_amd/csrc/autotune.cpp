@@ -93,6 +93,10 @@ extern "C" int bpx_autotune_batch(bpx_ctx* ctx, uint32_t num_queries, const uint
                                   &b);
         if (r == BPX_OK) r = time_candidate(ctx, b, db, first, count, c.launch, ms);
         bpx_batch_destroy(b);
+        // candidates size their result buffers differently: cached buffers
+        // of one are rarely reused by the next, so none are kept
+        const int rt = bpx_context_trim(ctx);
+        if (r == BPX_OK) r = rt;
         if (r == BPX_OK) record(c, *ms);
         if (log)
             std::fprintf(stderr,

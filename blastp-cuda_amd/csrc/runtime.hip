@@ -77,6 +77,14 @@ struct Pool {
         else
             (void)hipFree(p);
     }
+    // frees every cached (unused) allocation
+    void trim() {
+        for (auto& kv : free_) {
+            release(kv.second);
+            sizes_.erase(kv.second);
+        }
+        free_.clear();
+    }
     hipError_t alloc(void** p, size_t bytes) {
         return host_ ? hipHostMalloc(p, bytes, flags_) : hipMalloc(p, bytes);
     }
@@ -101,11 +109,7 @@ struct Pool {
             // allocation retried): clear it from the thread's last error,
             // which every kernel launch check reads
             (void)hipGetLastError();
-            for (auto& kv : free_) {
-                release(kv.second);
-                sizes_.erase(kv.second);
-            }
-            free_.clear();
+            trim();
             if (alloc(&p, bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 return nullptr;
@@ -1650,6 +1654,17 @@ int bpx_check_memory(bpx_ctx* ctx, const bpx_db* db, const bpx_query* q, uint32_
     if (rc) return rc;
     if (ctx->hbm_bytes && need > (uint64_t)ctx->hbm_bytes)
         return fail(BPX_E_BUDGET, "needs %llu device bytes", (unsigned long long)need);
+    return BPX_OK;
+}
+
+int bpx_context_trim(bpx_ctx* ctx) {
+    if (!ctx) return fail(BPX_E_ARG, "ctx is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (hipStream_t l : ctx->lanes) HIP_TRY(hipStreamSynchronize(l));
+    ctx->pool.trim();
+    ctx->pinned.trim();
+    ctx->coherent.trim();
     return BPX_OK;
 }
 

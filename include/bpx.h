@@ -175,6 +175,14 @@ int bpx_check_memory(bpx_ctx *ctx, const bpx_db *db, const bpx_query *q,
  * other work.  0 restores the device size.  The check of GPU_BLASTP_check_memory
  * (gpu_blastp.cu:316-333) against totalGlobalMem, with the total settable. */
 int bpx_context_limit(bpx_ctx *ctx, int64_t hbm_bytes);
+/* Frees the device and pinned host buffers the context keeps cached for
+ * later scans (results and queries hold theirs until destroyed), after the
+ * context's streams finish.  A context releases its cache by itself when an
+ * allocation fails; this returns the memory at once, for a GPU shared with
+ * other work.  The counterpart of GPU_BLASTP_free_memory
+ * (gpu_cpu_common.h:71), which frees a search's buffers after each volume;
+ * here buffers are reused across scans until trimmed. */
+int bpx_context_trim(bpx_ctx *ctx);
 /* Footprint of a query of qlen residues scanned alone against db, planned
  * without building its table (the table read from HBM, the placement every
  * query can take): its diagonal slots, the most wavefronts per workgroup it

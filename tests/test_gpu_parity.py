@@ -356,7 +356,8 @@ def test_refused_pool_allocation_is_retried_without_a_stale_error(monkeypatch):
     stale out-of-memory once failed an autotuner trial that way).  With
     BPX_POOL_FAIL_FIRST set, every device-pool allocation of a new context
     first asks HIP for 4 EiB: a plain and a batched scan run through that
-    path and still equal the oracle."""
+    path and still equal the oracle, also after bpx_context_trim has freed
+    the cache."""
     monkeypatch.setenv("BPX_POOL_FAIL_FIRST", "100000")
     c = bpx.Context(0)
     try:
@@ -364,7 +365,10 @@ def test_refused_pool_allocation_is_retried_without_a_stale_error(monkeypatch):
         spec = dict(seed=9, num_subjects=1500, len_median=300, len_sigma=0.6, len_min=30, len_max=3000)
         sdb = bpx.synth_db(spec, [q], nthreads=THREADS)
         res, offs, lens = sdb.residues, sdb.offsets, sdb.lengths
-        assert_parity(gpu_run(c, q, res, offs, lens), oracle_run(q, res, offs, lens))
+        orc = oracle_run(q, res, offs, lens)
+        assert_parity(gpu_run(c, q, res, offs, lens), orc)
+        c.trim()  # the cache freed: the next scan allocates afresh
+        assert_parity(gpu_run(c, q, res, offs, lens), orc)
         queries = [bpx.synth_query(26, i, L) for i, L in enumerate([40, 120, 300, 700, 1500])]
         assert_batch_parity(c, queries, batch_db(queries, n=800, seed=36), group_residues=1000, group_queries=4)
     finally:
