@@ -525,8 +525,8 @@ def main():
         args.warmup = 1 if args.workload in ("headline", "skewed") else 3
     os.environ.setdefault("BPX_TUNE_LOG", "1")
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder shards by "
-              "subject across GPUs, but this build measures the single-GPU path.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the word finder would shard "
+              "by subject across GPUs, but the library and this benchmark drive a single GPU.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe > 0:
