@@ -673,6 +673,11 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
         const uint16_t* starts16 = reinterpret_cast<const uint16_t*>(starts32);
         const uint16_t* entries = narrow_starts ? starts16 + ((g.npresent + 2u) & ~1u)
                                                 : reinterpret_cast<const uint16_t*>(starts32 + g.npresent + 1);
+        // a table in HBM is looked up through its start offsets by word
+        // (one load for a word's range instead of presence word, rank and
+        // starts in turn); words at or past wlast read an empty range
+        const uint32_t* direct = p.tables + g.direct_word;
+        const uint32_t wlast = p.ngroups_pv * 32u;
         const QueryInfo q0 = qinfo[0];
         // flat: the descriptors of the claimed chunk, lane j holding those of
         // its j-th subject -- loaded with the claim, so a subject's residue
@@ -1085,9 +1090,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             bool empty_ring = false, full_drain = false;
 
             // The lookup of positions pos .. pos + 63 (one per lane) in four
-            // stages, each waiting for the previous one's LDS reads: the
-            // word's residues; its presence bits and rank base; its cell's
-            // start offsets; the hits into the ring.
+            // stages, each waiting for the previous one's reads: the word's
+            // residues; its presence bits and rank base (a table in HBM: its
+            // cell's range, read by word); its cell's start offsets (LDS
+            // tables only); the hits into the ring.
             struct Look {
                 uint32_t b[WORD];
                 uint32_t wd, start, end;
@@ -1114,9 +1120,16 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 // sent to (27, 31)
                 if (WORD == 2 && L.b[0] >= 28u) wd = (27u << 5) | 31u;
                 L.wd = wd;
-                L.gv = pvr[min(wd >> 5, p.ngroups_pv - 1u)];
+                if (TAB_LDS) {
+                    L.gv = pvr[min(wd >> 5, p.ngroups_pv - 1u)];
+                } else {
+                    const uint32_t wi = min(wd, wlast);
+                    L.start = direct[wi];
+                    L.end = direct[wi + 1];
+                }
             };
             auto look_c = [&](Look& L) {
+                if (!TAB_LDS) return;  // the range is read by word
                 const uint32_t bit = 1u << (L.wd & 31u);
                 // rank of the word among present words; an absent word reads
                 // the neighbouring (valid) start entries and is masked out
@@ -1133,7 +1146,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // appends the hits of as many leading positions as the ring holds
             auto look_d = [&](Look& L) {
                 const int32_t s = pos + (int32_t)lane;
-                const bool present = (L.gv.x & (1u << (L.wd & 31u))) && s <= last;
+                const bool present = (!TAB_LDS || (L.gv.x & (1u << (L.wd & 31u))) != 0) && s <= last;
                 const uint32_t cnt = present ? L.end - L.start : 0u;
                 const uint32_t incl = wave_inclusive_sum(cnt);
                 const bool fits = incl <= qcap - count;

@@ -34,7 +34,8 @@ struct GroupInfo {
     uint32_t nq;           // queries in the group
     uint32_t diag_slots;   // sum of the members' diagonal-table lengths
     uint32_t flags;        // kGroupNarrowStarts: u16 start offsets
-    uint32_t pad[3];
+    uint32_t direct_word;  // offset (in u32 words, same arena) of its start offsets by word
+    uint32_t pad[2];
 };
 constexpr uint32_t kGroupNarrowStarts = 1;
 

@@ -323,6 +323,7 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
         members.push_back(std::move(m));
     }
     std::vector<uint32_t> tables;
+    std::vector<std::vector<uint32_t>> directs;
     std::vector<uint8_t> qres, qmap;
     for (const auto& m : members) {
         std::vector<bpx::QuerySpan> spans;
@@ -359,9 +360,18 @@ int build_set(bpx_ctx* ctx, const std::vector<QuerySpec>& qs, const int8_t* M, i
         gi.npresent = t.npresent;
         gi.flags = t.narrow_starts() ? bpx::kGroupNarrowStarts : 0u;
         tables.insert(tables.end(), blob.begin(), blob.end());
+        directs.push_back(t.direct_starts());
         set.max_cell.push_back(t.max_cell);
         set.groups.push_back(gi);
         if (keep_tables) keep_tables->push_back(std::move(t));
+    }
+    // each group's start offsets by word, after the packed tables (a
+    // launch that reads its table from HBM looks words up through these)
+    for (size_t gi = 0; gi < directs.size(); ++gi) {
+        if (tables.size() + directs[gi].size() > 0xFFFFFFFFull)
+            return fail(BPX_E_BUDGET, "query tables exceed 2^32 words; use fewer queries per batch");
+        set.groups[gi].direct_word = (uint32_t)tables.size();
+        tables.insert(tables.end(), directs[gi].begin(), directs[gi].end());
     }
     set.table_bytes = (uint64_t)tables.size() * 4;
     // zero words after the last table: the scan reads start entries at a

@@ -178,6 +178,18 @@ uint64_t HostTable::packed_bytes() const {
     return 8ull * packed_groups() + st + e;
 }
 
+std::vector<uint32_t> HostTable::direct_starts() const {
+    const uint32_t W = packed_groups() * 32u;
+    std::vector<uint32_t> d((size_t)W + 2);
+    uint32_t r = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        d[w] = starts[r];
+        if ((groups[2 * (w >> 5)] >> (w & 31)) & 1u) ++r;
+    }
+    d[W] = d[(size_t)W + 1] = starts[npresent];
+    return d;
+}
+
 std::vector<uint32_t> HostTable::pack() const {
     std::vector<uint32_t> blob(packed_bytes() / 4, 0);
     uint8_t* b = reinterpret_cast<uint8_t*>(blob.data());

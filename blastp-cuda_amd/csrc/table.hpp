@@ -51,6 +51,13 @@ struct HostTable {
     uint32_t packed_groups() const;
     std::vector<uint32_t> pack() const;
     uint64_t packed_bytes() const;
+    // Start offsets indexed by word over the packed word range W
+    // (packed_groups() * 32), W + 2 entries: word w's query offsets are
+    // entries[d[w] .. d[w+1]), empty for an absent word, and d[W] = d[W+1] =
+    // the entry count, so any word at or past W reads an empty range.  A
+    // table read from HBM is looked up through this array in one step
+    // instead of presence word, then rank, then starts.
+    std::vector<uint32_t> direct_starts() const;
 };
 
 // Builds the neighbourhood table of `query` (ncbistdaa, qlen >= 0): word w is
