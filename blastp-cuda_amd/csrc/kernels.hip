@@ -1092,12 +1092,13 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
             // The lookup of positions pos .. pos + 63 (one per lane) in four
             // stages, each waiting for the previous one's reads: the word's
             // residues; its presence bits and rank base (a table in HBM: its
-            // cell's range, read by word); its cell's start offsets (LDS
-            // tables only); the hits into the ring.
+            // cell's range, read by word); its cell's start offsets (a table
+            // in HBM: its first eight entries); the hits into the ring.
             struct Look {
                 uint32_t b[WORD];
                 uint32_t wd, start, end;
                 uint2 gv;
+                uint32_t pe[8];  // a table in HBM: the cell's first entries
             };
             auto look_a = [&](Look& L) {
                 const int32_t s = pos + (int32_t)lane;
@@ -1129,7 +1130,16 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                 }
             };
             auto look_c = [&](Look& L) {
-                if (!TAB_LDS) return;  // the range is read by word
+                if (!TAB_LDS) {
+                    // the range was read by word: the cell's first eight
+                    // entries go out now, so their memory round trip
+                    // overlaps the rest of the claim rounds (most cells hold
+                    // no more).  Loads past the cell stay inside the table
+                    // buffer, as in the ring append.
+#pragma unroll
+                    for (uint32_t k = 0; k < 8; ++k) L.pe[k] = entries[L.start + k];
+                    return;
+                }
                 const uint32_t bit = 1u << (L.wd & 31u);
                 // rank of the word among present words; an absent word reads
                 // the neighbouring (valid) start entries and is masked out
@@ -1167,12 +1177,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_scan(ScanParams p) {
                         for (uint32_t t = 0; t < cnt; ++t)
                             ring[(at + t) & qmask] = ((uint32_t)entries[L.start + t] << 16) | sv;
                     } else {
-                        // a table in HBM: eight independent loads per step, so a
-                        // cell costs one memory round trip per eight entries
-                        // instead of one per entry.  Loads past the cell stay
-                        // inside the table buffer (the next group's table, or
-                        // the zero words after the last one).
-                        for (uint32_t t = 0; t < cnt; t += 8) {
+                        // a table in HBM: the first eight entries were loaded
+                        // during the claim rounds; the rest go out eight
+                        // independent loads per step, so a cell costs one
+                        // memory round trip per eight entries instead of one
+                        // per entry.  Loads past the cell stay inside the
+                        // table buffer (the next group's table, or the start
+                        // offsets by word after the last one).
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k)
+                            if (k < cnt) ring[(at + k) & qmask] = (L.pe[k] << 16) | sv;
+                        for (uint32_t t = 8; t < cnt; t += 8) {
                             uint32_t v[8];
 #pragma unroll
                             for (uint32_t k = 0; k < 8; ++k) v[k] = entries[L.start + t + k];
