@@ -222,13 +222,11 @@ extern "C" int bpx_table_build(const uint8_t* query, int32_t qlen, const int8_t*
     if (num_entries) *num_entries = t.entries.size();
     if (!cell_start) return BPX_E_ARG;
     if (capacity < t.entries.size()) return BPX_E_ARG;
-    uint32_t r = 0;
-    for (uint32_t w = 0; w < t.nwords; ++w) {
-        const bool present = (t.groups[2 * (w >> 5)] >> (w & 31)) & 1u;
-        cell_start[w] = t.starts[r];
-        if (present) ++r;
-    }
-    cell_start[t.nwords] = t.starts[t.npresent];
+    // the start offsets by word the scan reads for tables in HBM, extended
+    // over the words it drops (first residue code >= 28: never present)
+    const std::vector<uint32_t> d = t.direct_starts();
+    const uint32_t W = (uint32_t)d.size() - 2;
+    for (uint32_t w = 0; w <= t.nwords; ++w) cell_start[w] = d[std::min(w, W)];
     if (entries && !t.entries.empty()) std::memcpy(entries, t.entries.data(), t.entries.size() * 2);
     return BPX_OK;
 }
