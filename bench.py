@@ -13,6 +13,9 @@ over the 1,024 queries.  Beside it, in `config`:
     reference launches one query at a time -- and batched_over_plain;
   * the same batch under the static longest-first order, and the learned
     cost order's gain over it;
+  * split_by_length: batched vs each-query-alone kernel time for the
+    queries up to 1,000 aa (packed into groups) and the longer ones (groups
+    of one), which shows where batched_over_plain is set;
   * the plain 500-aa query against the 2,000,000-subject database
     (bpx.workloads 2).
 `roofline` is that of the batched step's k_scan launches (every launch
@@ -431,10 +434,34 @@ def headline(args):
     plain_value, plain_times = median_rate(plain_pass, work, 3)
     if plain_records[0] != nrec:
         raise SystemExit(f"batched and plain record counts differ: {nrec} vs {plain_records[0]}")
+    # where the ratio is set, outside the timed passes: each query's kernel
+    # time alone, and one batched scan (longest-first order, the tuned policy
+    # and launch) of the queries up to 1,000 aa and one of the longer ones
+    alone_ms = np.zeros(len(dqs))
+    for i, dq in enumerate(dqs):
+        r = bpx.scan(ctx, dq, ddb, subject_stats=False).wait()
+        alone_ms[i] = r.timing()["kernel_ms"]
+        r.close()
     for dq in dqs:
         dq.close()
+    split = {}
+    for label, pick in (("queries_up_to_1000_aa", lambda L: L <= 1000), ("queries_over_1000_aa", lambda L: L > 1000)):
+        idx = [i for i, q in enumerate(qs) if pick(len(q))]
+        if not idx:
+            continue
+        bs = ctx.prepare_batch([qs[i] for i in idx], **policy)
+        kb = []
+        for _ in range(2):
+            r = bpx.scan_batch(ctx, bs, ddb, launch=launch).wait()
+            kb.append(r.timing()["kernel_ms"])
+            r.close()
+        ng = bs.info()["groups"]
+        bs.close()
+        alone = float(alone_ms[idx].sum())
+        split[label] = {"queries": len(idx), "groups": ng, "batched_kernel_ms": round(min(kb), 1),
+                        "alone_kernel_ms": round(alone, 1), "ratio": round(alone / min(kb), 3)}
     log(f"  plain (each query alone): {np.median(plain_times):.2f} s -> {plain_value / 1e9:.2f} G res/s; "
-        f"batched over plain {value / plain_value:.3f}")
+        f"batched over plain {value / plain_value:.3f}; by query length (kernel ms): {split}")
     b.close()
     ddb.close()
 
@@ -485,6 +512,7 @@ def headline(args):
                       "ms_longest_first": round(ms_lf, 3), "steps_longest_first": lf_steps,
                       "gain_over_longest_first": round(ms_lf / ms, 4),
                       "cost_model_train_s": round(train_s, 2), "cost_model_inference_s": round(infer_s, 3),
+                      "split_by_length": split,
                       "phases": batched_phases, "plain_2M": plain2},
            "roofline": roofline,
            "roofline_issue": issue_roofline("sq_counters_k_scan_batched_1024_queries_500k_subjects", kern,
